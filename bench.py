@@ -1,0 +1,163 @@
+"""Benchmark: FP64 dynamic-aperture scan on one MI355X.
+
+Workload (one "step"): StableAperture over the 16-period DBA ring (400
+elements, teapot KICK, FIVED, dp = 0) for a 1024 x 1024 grid of initial
+(x, y), x in [-20, 20] mm, y in [0, 8] mm, 1024 turns, aperture |x|,|y| <
+30 mm.  The particle pool is resident in HBM before the timed region.
+
+Metric: particle-element-turns per second, PET = sum over particles of the
+turns executed (loss turn + 1, or all turns) x E, E = 400 elements (markers
+and apertures excluded).
+
+Also reported:
+  roofline   the tracking kernel's canonical FP64 work per launch (executed
+             particle-turns x 13952 FLOP, the UFO FLOP rule) over its average
+             launch time from HIP events on the launch stream, against the
+             MI355X FP64 peak of 78.6 TFLOP/s;
+  cpu_baseline  the oracle (the reference's kernels restated in C, gcc
+             -O3 -ffast-math, OpenMP over all host threads) on an evenly
+             strided sample of the same grid, as PET/s.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_FP64_TFLOPS = 78.6
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--nx", type=int, default=1024)
+    ap.add_argument("--ny", type=int, default=1024)
+    ap.add_argument("--turns", type=int, default=1024)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    import ufo_amd as ufo
+    from ufo_amd import lattices
+
+    ring = lattices.dba_ring(16)
+    flags = ufo.KICK | ufo.DOUBLE_PRECISION | ufo.FIVED
+    pool = lattices.da_grid(args.nx, args.ny)
+    n = len(pool)
+    t0 = time.perf_counter()
+    sa = ufo.StableAperture(ring, flags=flags, turns=args.turns, particles=n, parameters=["x", "y"])
+    sa.parameters[:] = pool
+    sa.module()
+    compile_s = time.perf_counter() - t0
+    sa.pool_device.copy_(sa._pool_host)
+    torch.cuda.synchronize()
+    E = sa.kernel.elements
+    flop_pt = sa.kernel.flops_per_turn
+
+    for _ in range(args.warmup):
+        sa.launch()
+    torch.cuda.synchronize()
+
+    stream = torch.cuda.current_stream()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2 * args.steps)]
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()
+    for k in range(args.steps):
+        ev[2 * k].record(stream)
+        sa.launch()
+        ev[2 * k + 1].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t_start
+    kernel_ms = [ev[2 * k].elapsed_time(ev[2 * k + 1]) for k in range(args.steps)]
+
+    lost = sa.lost_device.cpu().numpy().view(np.uint32)
+    executed = np.minimum(lost.astype(np.int64) + 1, args.turns)
+    pt = int(executed.sum())
+    pet_step = pt * E
+    ms_step = wall / args.steps * 1e3
+    value = pet_step * args.steps / wall
+    avg_kernel_s = float(np.mean(kernel_ms)) / 1e3
+    achieved_tflops = pt * flop_pt / avg_kernel_s / 1e12
+
+    info = sa.module().info(sa.kernel.name)
+    out = {
+        "metric": "FP64 particle-element-turns/s (DA scan, StableAperture)",
+        "value": value,
+        "unit": "particle-element-turns/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": "dynamic-aperture scan: 16-period DBA ring (400 elements, teapot KICK, FIVED), "
+                        f"{args.nx}x{args.ny} grid x in [-20,20] mm, y in [0,8] mm, {args.turns} turns",
+            "particles": n, "turns": args.turns, "elements": E,
+            "flop_per_particle_turn": flop_pt,
+            "executed_particle_turns": pt,
+            "survivors": int(np.sum(lost == args.turns)),
+            "hiprtc_compile_s": round(compile_s, 3),
+            "lanes": sa.lanes, "vgprs": info["vgprs"],
+            "device": repr(sa.ctx.device),
+        },
+        "roofline": {
+            "bound": "valu-fp64",
+            "achieved": achieved_tflops,
+            "peak": PEAK_FP64_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": achieved_tflops / PEAK_FP64_TFLOPS,
+            "traffic": None,
+            "kernel": sa.kernel.name,
+            "kernel_ms": float(np.mean(kernel_ms)),
+        },
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(ring, flags, pool, args.turns, E, args.cpu_seconds)
+    print(json.dumps(out))
+
+
+def cpu_baseline(ring, flags, pool, turns, E, budget_s):
+    """The oracle (restated reference kernels, gcc fast-math build, OpenMP)
+    on an evenly strided sample of the grid, sized to ~budget_s seconds."""
+    import numpy as np
+    from oracle import runner
+    threads = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
+    probe = pool[:: max(1, len(pool) // 256)][:256]
+    t = time.perf_counter()
+    runner.stable_aperture(ring, probe, flags=flags, turns=turns, parameters=["x", "y"], mode="fast")
+    t = time.perf_counter()
+    lost = runner.stable_aperture(ring, probe, flags=flags, turns=turns, parameters=["x", "y"], mode="fast")
+    dt = time.perf_counter() - t
+    per_particle = dt / len(probe)
+    m = int(min(len(pool), max(256, budget_s / max(per_particle, 1e-9))))
+    stride = max(1, len(pool) // m)
+    sample = pool[::stride][:m]
+    t = time.perf_counter()
+    lost = runner.stable_aperture(ring, sample, flags=flags, turns=turns, parameters=["x", "y"], mode="fast")
+    dt = time.perf_counter() - t
+    pt = int(np.minimum(lost.astype(np.int64) + 1, turns).sum())
+    return {"value": pt * E / dt, "unit": "particle-element-turns/s", "cores": threads, "kind": "port",
+            "sample": f"{len(sample)} grid points (every {stride}th of the {len(pool)}-point grid), "
+                      f"{turns} turns, {dt:.1f} s"}
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,197 @@
+"""GPU parity: HIP kernels against the oracle on the same seeded inputs.
+
+Every test calls the product (ufo_amd.Track / StableAperture / Optics), which
+generates HIP source, compiles it with hipRTC through libufo_hip.so and
+launches it on cuda:0; the oracle (gcc build of the restated reference
+kernels) is the checker.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import ufo_amd as ufo
+from ufo_amd import lattices
+from oracle import runner
+
+from conftest import OPTICS_DIR
+from parity import REL_TOL, compare_tracks, rel_err
+
+pytestmark = pytest.mark.gpu
+
+COORD6 = ["x", "px", "y", "py", "dp"]
+
+
+def _track_both(line, pool, flags, turns, parameters, where, dp=0.0):
+    tr = ufo.Track(line, flags=flags, turns=turns, particles=len(pool), parameters=parameters, where=where, dp=dp)
+    tr.parameters[:] = pool
+    tr.run()
+    gpu = np.array(tr.tracks)
+    strict = runner.track(line, pool, flags=flags, turns=turns, parameters=parameters, where=where, dp=dp)
+    fast = runner.track(line, pool, flags=flags, turns=turns, parameters=parameters, where=where, dp=dp,
+                        mode="fast")
+    return gpu, strict, fast
+
+
+def test_fodo_track_6d():
+    """FODO-16 (4096-particle workload at reduced size), 6D, dp per particle."""
+    ring = lattices.fodo_ring(16)
+    bunch = lattices.gaussian_bunch(512, seed=0)
+    bunch[:8, 0] = np.linspace(0.03, 0.06, 8)      # some lost at the aperture
+    gpu, strict, fast = _track_both(ring, bunch, ufo.KICK | ufo.DOUBLE_PRECISION, 200, COORD6, [-1])
+    worst, reg, flags_ok = compare_tracks(gpu, strict, fast)
+    print(f"regular {reg.sum()}/{len(reg)}, worst rel err {worst:.3e}")
+    assert flags_ok
+    assert np.array_equal(np.isnan(gpu[:, :, 0]), np.isnan(strict[:, :, 0]))
+    assert reg.sum() > 400
+    assert worst <= REL_TOL
+    # dp and z columns are carried unchanged
+    alive = ~np.isnan(gpu[:, -1, 0])
+    assert np.all(gpu[alive, :, 5] == bunch[alive, None, 4])
+    assert np.all(gpu[alive, :, 4] == 0.0)
+
+
+def test_dba_stable_aperture_subset():
+    """DA ring, a 32x8 subset of the grid (including the lossy corner),
+    1024 turns: identical loss turns on regular particles."""
+    ring = lattices.dba_ring(16)
+    flags = ufo.KICK | ufo.DOUBLE_PRECISION | ufo.FIVED
+    pool = lattices.da_grid(32, 8)
+    sa = ufo.StableAperture(ring, flags=flags, turns=1024, particles=len(pool), parameters=["x", "y"])
+    sa.parameters[:] = pool
+    sa.run()
+    strict = runner.stable_aperture(ring, pool, flags=flags, turns=1024, parameters=["x", "y"])
+    fast = runner.stable_aperture(ring, pool, flags=flags, turns=1024, parameters=["x", "y"], mode="fast")
+    regular = strict == fast
+    print(f"lost: {np.sum(strict < 1024)}; strict/fast agree on {regular.sum()}/{len(regular)}")
+    assert np.array_equal(sa.lost[regular], strict[regular])
+
+
+def test_dba_track_coordinates():
+    """DA ring tracked 256 turns from grid points: coordinates within 1e-12
+    of the oracle on regular particles."""
+    ring = lattices.dba_ring(16)
+    flags = ufo.KICK | ufo.DOUBLE_PRECISION | ufo.FIVED
+    pool = lattices.da_grid(16, 4)
+    gpu, strict, fast = _track_both(ring, pool, flags, 256, ["x", "y"], [-1])
+    worst, reg, flags_ok = compare_tracks(gpu, strict, fast)
+    print(f"regular {reg.sum()}/{len(reg)}, worst rel err {worst:.3e}")
+    assert flags_ok and reg.sum() >= len(reg) // 2 and worst <= REL_TOL
+
+
+def test_alba_thick_track_fp64():
+    """ALBA ring with thick quads, sector bends with pole faces, sextupoles:
+    50 turns, FIVED, dp = 1e-3 literal."""
+    alba = ufo.Lattice(os.path.join(OPTICS_DIR, "alba.mad"))
+    rng = np.random.default_rng(1)
+    pool = np.stack([rng.normal(0, 2e-3, 64), rng.normal(0, 1e-4, 64),
+                     rng.normal(0, 1e-3, 64), rng.normal(0, 1e-4, 64)], 1)
+    flags = ufo.DOUBLE_PRECISION | ufo.FIVED
+    gpu, strict, fast = _track_both(alba.RING, pool, flags, 50, ["x", "px", "y", "py"], [-1], dp=1e-3)
+    worst, reg, flags_ok = compare_tracks(gpu, strict, fast)
+    print(f"regular {reg.sum()}/{len(reg)}, worst rel err {worst:.3e}")
+    assert flags_ok and reg.sum() >= 32 and worst <= REL_TOL
+
+
+def _all_elements_line():
+    q = ufo.Quadrupole("Q", length=0.6, k1=1.1, slices=5, dx=2e-4, dy=-1e-4, dkn=[0, 0, 3.0], dks=[0, 0.01, 0, 8.0])
+    b = ufo.Sbend("B", length=1.2, angle=0.15, k1=-0.3, e1=0.05, e2=0.07, hgap=0.02, fint=0.5, slices=6)
+    r = ufo.Rbend("R", length=0.9, angle=0.1, k1=0.0, slices=3)
+    s = ufo.Sextupole("S", length=0.3, k2=20.0, k2s=5.0, dx=1e-4, dkn=[0, 0, 0, 4.0])
+    o = ufo.Octupole("O", length=0.2, k3=300.0, slices=2)
+    m = ufo.Multipole("M", knl=[1e-5, 0.02, 1.5, 40.0, 800.0, 3e4], ksl=[2e-5, 0.01, 0.5, 10.0, 0, 1e4],
+                      dx=1e-4, dy=2e-4)
+    w = ufo.Wire("W", x=0.05, y=0.03, k=1e-6)
+    d1, d2 = ufo.Drift("D1", length=0.7), ufo.Drift("D2", length=1.3)
+    qd = ufo.Quadrupole("QD", length=0.5, k1=-1.3, slices=4)
+    ap = ufo.Aperture("AP", window="(fabs(x) < 0.02 && fabs(y) < 0.02)")
+    mk = ufo.Marker("MK")
+    cell = ufo.Line("C", [q, d1, b, d2, s, mk, qd, d1, o, m, r, d2, w, ap])
+    return ufo.Line("RING", [cell] * 3)
+
+
+@pytest.mark.parametrize("flags", [
+    ufo.KICK | ufo.DOUBLE_PRECISION,
+    ufo.KICK | ufo.DOUBLE_PRECISION | ufo.EXACT,
+    ufo.KICK | ufo.DOUBLE_PRECISION | ufo.FIVED,
+    ufo.DOUBLE_PRECISION,
+    ufo.DOUBLE_PRECISION | ufo.ACHROMATIC,
+    ufo.KICK | ufo.DOUBLE_PRECISION | ufo.LINEAR,
+])
+def test_every_element_kind(flags):
+    """Every element class, misalignments, multipole errors, pole faces,
+    fringe, wire, aperture, with observation points inside elements."""
+    line = _all_elements_line()
+    rng = np.random.default_rng(7)
+    n = 128
+    pool = np.stack([rng.normal(0, 2e-3, n), rng.normal(0, 2e-4, n), rng.normal(0, 2e-3, n),
+                     rng.normal(0, 2e-4, n), rng.normal(0, 1e-3, n)], 1)
+    where = [0.5, 2, 2.25, 4.5, 11, -1]
+    gpu, strict, fast = _track_both(line, pool, flags, 40, COORD6, where, dp=2e-3)
+    worst, reg, flags_ok = compare_tracks(gpu, strict, fast)
+    print(f"flags {flags}: regular {reg.sum()}/{n}, worst rel err {worst:.3e}")
+    assert flags_ok and reg.sum() >= n // 2 and worst <= REL_TOL
+
+
+def test_lattice_parameters_bound_per_particle():
+    """Lattice values as per-particle parameters (k1 of a family, a dkN
+    error, a displacement, a multipole kN, a length), tracked in one launch."""
+    ring = lattices.fodo_ring(8)
+    prm = ["x", "y", ("QF", "k1"), ("QD", "dk2"), ("SF", "k2"), ("QF", "dx"), (2, "length")]
+    n = 96
+    rng = np.random.default_rng(3)
+    pool = np.stack([rng.normal(0, 1e-3, n), rng.normal(0, 1e-3, n), 1.2 + rng.normal(0, 0.02, n),
+                     rng.normal(0, 0.5, n), 0.8 + rng.normal(0, 0.1, n), rng.normal(0, 1e-4, n),
+                     2.0 + rng.normal(0, 0.01, n)], 1)
+    flags = ufo.KICK | ufo.DOUBLE_PRECISION | ufo.FIVED
+    gpu, strict, fast = _track_both(ring, pool, flags, 100, prm, [-1])
+    worst, reg, flags_ok = compare_tracks(gpu, strict, fast)
+    assert flags_ok and reg.sum() >= n // 2 and worst <= REL_TOL
+
+
+def test_fp32_thick_madx_vector():
+    """The reference's default precision (FP32, thick): MAD-X vector of
+    tests/tracking.py within its 1e-6 tolerance."""
+    lat = ufo.Lattice(os.path.join(OPTICS_DIR, "fodo.mad"))
+    tr = ufo.Track(lat.RING, turns=10, particles=2, where=[-1], flags=ufo.FIVED, parameters=["x"])
+    tr.parameters[:, 0] = 0.001
+    tr.run()
+    x = np.array([-0.0008304642239, -0.001747306891, -0.0007418783837, 0.001079715459, 0.001713478007,
+                  0.000462185744, -0.00129757264, -0.001629827644, -0.0001690544829, 0.001477701296])
+    assert np.max(np.abs(tr.tracks[0][:, 0] - x)) < 1e-6
+    assert np.max(np.abs(tr.tracks[1][:, 0] - x)) < 1e-6
+
+
+def test_optics_alba_tunes():
+    """ALBA periodic tunes and Twiss vs MAD-X (tests/optics.py) and vs the
+    oracle's Optics."""
+    alba = ufo.Lattice(os.path.join(OPTICS_DIR, "alba.mad"))
+    where = alba.RING.find(lambda e: isinstance(e, ufo.Quadrupole))[:20]
+    fl = ufo.LINEAR | ufo.ACHROMATIC | ufo.DOUBLE_PRECISION
+    o = ufo.Optics(alba.RING, where=list(where), flags=fl)
+    o.run()
+    assert abs(o.qx - 0.15543311) < 1e-8 and abs(o.qy - 0.362008337) < 1e-8
+    r = runner.Optics(alba.RING, where=list(where), flags=fl)
+    r.run()
+    for a in ("bx", "by", "ax", "ay", "dx", "dpx", "mux", "muy"):
+        np.testing.assert_allclose(getattr(o, a), getattr(r, a), rtol=1e-9, atol=1e-12)
+
+
+def test_sa_threads_and_refill():
+    """Results do not depend on the number of resident lanes (persistent
+    refill) — run with 64 lanes and with the full device."""
+    ring = lattices.fodo_ring(16)
+    flags = ufo.KICK | ufo.DOUBLE_PRECISION | ufo.FIVED
+    pool = lattices.da_grid(40, 40, xlim=0.05, ymax=0.05)
+    sa = ufo.StableAperture(ring, flags=flags, turns=300, particles=len(pool), parameters=["x", "y"])
+    sa.parameters[:] = pool
+    sa.run()
+    full = sa.lost.copy()
+    sa._pool_dev.copy_(sa._pool_host)
+    sa.launch(lanes=64)
+    sa._lost_host.copy_(sa._lost_dev)
+    import torch
+    torch.cuda.synchronize()
+    assert np.array_equal(sa.lost, full)
+    ref = runner.stable_aperture(ring, pool, flags=flags, turns=300, parameters=["x", "y"])
+    assert np.array_equal(full, ref)

@@ -1,0 +1,654 @@
+"""HIP source emitter: IR ops -> kernels compiled by hipRTC for gfx950.
+
+Kernel shape (all kinds):
+  * one particle per lane, wave64; the particle state x, px, y, py lives in
+    FP64 (or FP32) VGPRs, z/dp/oodppo and every per-particle invariant (an
+    expression of parameters and oodppo: chromatic kick strengths, thick
+    element matrices, edge kicks) are computed once per particle and kept in
+    registers;
+  * element constants are folded on the host and emitted as literals, so they
+    end up as SGPR constants of the ISA;
+  * arithmetic is written with explicit ``fma`` and compiled with
+    ``-ffp-contract=off``: what is emitted is what runs;
+  * a turn whose op list is a repetition of one period (a ring of identical
+    cells) is emitted as a loop over the period, keeping the code within the
+    instruction cache;
+  * HBM traffic only at particle start (one row of the parameter pool) and at
+    particle end (the loss turn), or at the observation points of Track.
+
+Kernel kinds:
+  ``sa``      StableAperture (ufo/sa.py:91-126).  Persistent lanes with
+              refill: a lane whose particle is lost (or survives all turns)
+              stores the result and claims the next particle from a
+              wave-aggregated atomic counter (the reference's
+              ``atomic_inc(pool_idx)`` work pool, ufo/sa.py:122), so lanes do
+              not idle behind the longest-living particle of their wave.
+  ``track``   Track (ufo/track.py:84-114), one lane per particle, samples
+              stored structure-of-arrays [sample][coordinate][particle] so the
+              stores of a wave are coalesced.
+  ``optics``  Optics (ufo/optics.py:87-115): the three-ray linear-map kernel.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+from typing import Dict, List, Optional
+
+import numpy as np
+
+from . import ir
+from .flags import FIVED, DOUBLE_PRECISION
+from .ir import Expr, fold, is_zero
+
+
+COORDS = ("x", "px", "y", "py", "z", "dp")
+
+
+# ---------------------------------------------------------------------------
+# Expression printing
+# ---------------------------------------------------------------------------
+
+def literal(v: float, fp32: bool) -> str:
+    if math.isnan(v):
+        return "__builtin_nanf(\"\")" if fp32 else "__builtin_nan(\"\")"
+    if math.isinf(v):
+        s = "__builtin_inff()" if fp32 else "__builtin_inf()"
+        return s if v > 0 else f"(-{s})"
+    s = repr(float(np.float32(v))) if fp32 else repr(float(v))
+    if "e" not in s and "." not in s:
+        s += ".0"
+    return f"{s}f" if fp32 else s
+
+
+@dataclass
+class Env:
+    """Code-generation context of one kernel."""
+    fp32: bool
+    consts: Dict[str, float]                 # kernel variables known at generation (dp, oodppo)
+    invariants: Dict[Expr, str] = field(default_factory=dict)
+    inv_code: List[str] = field(default_factory=list)
+    params_used: set = field(default_factory=set)
+    tmp: int = 0
+
+    def f(self, e) -> Expr:
+        return fold(e, self.fp32, self.consts)
+
+    def lit(self, v: float) -> str:
+        return literal(v, self.fp32)
+
+    def raw(self, e: Expr) -> str:
+        """C text of an expression, without hoisting."""
+        t = e.t
+        k = t[0]
+        if k == "c":
+            return self.lit(t[1])
+        if k == "p":
+            self.params_used.add(t[1])
+            return f"P{t[1]}"
+        if k == "v":
+            return t[1]
+        if k == "r":
+            return f"({t[1]})"
+        if k == "neg":
+            return f"(-{self.raw(t[1])})"
+        if k == "f":
+            return f"{t[1]}({self.raw(t[2])})"
+        if k == "?":
+            return f"({self.raw(t[1])} ? {self.raw(t[2])} : {self.raw(t[3])})"
+        return f"({self.raw(t[1])} {k} {self.raw(t[2])})"
+
+    def val(self, e) -> str:
+        """C text of a folded expression; non-constant expressions that do
+        not depend on the turn are hoisted into per-particle invariants."""
+        e = self.f(e)
+        k = e.t[0]
+        if k in ("c", "v"):
+            return self.raw(e)
+        if k == "p":
+            return self.raw(e)
+        if e.has_raw() or e.uses_var("turn"):
+            return self.raw(e)
+        if k == "neg":
+            return f"(-{self.val(e.t[1])})"
+        name = self.invariants.get(e)
+        if name is None:
+            name = f"I{len(self.invariants)}"
+            self.invariants[e] = name
+            self.inv_code.append(f"{name} = {self.raw(e)};")
+        return name
+
+    def neg(self, e) -> str:
+        e = self.f(e)
+        if e.is_const():
+            return self.lit(-e.value)
+        return self.val(-e)
+
+    def new_tmp(self) -> str:
+        self.tmp += 1
+        return f"t{self.tmp}"
+
+
+# ---------------------------------------------------------------------------
+# Forward pass of each op
+# ---------------------------------------------------------------------------
+
+def _fma(a, b, c):
+    return f"fma({a}, {b}, {c})"
+
+
+def emit_op(op, env: Env, out: List[str], observe=None):
+    f = env.f
+    if isinstance(op, ir.Drift):
+        L = f(op.L)
+        if is_zero(L):
+            return
+        v = env.val(L)
+        out.append(f"x = {_fma('px', v, 'x')}; y = {_fma('py', v, 'y')};")
+    elif isinstance(op, ir.ExactDrift):
+        L = env.val(op.L)
+        opdp2 = env.val(ir.E(1.0 + ir.DP) * (1.0 + ir.DP))
+        t = env.new_tmp()
+        out.append(f"{{ ufloat {t} = {L} / sqrt({_fma('-py', 'py', _fma('-px', 'px', opdp2))}); "
+                   f"x = {_fma('px', t, 'x')}; y = {_fma('py', t, 'y')}; }}")
+    elif isinstance(op, ir.Kick):
+        emit_kick(op, env, out)
+    elif isinstance(op, ir.Weak):
+        w = f(op.w)
+        if not is_zero(w):
+            out.append(f"px = {_fma(env.neg(w), 'x', 'px')};")
+    elif isinstance(op, ir.Shift):
+        dx, dy = f(op.dx), f(op.dy)
+        if not is_zero(dx):
+            out.append(f"x = x - {env.val(dx)};")
+        if not is_zero(dy):
+            out.append(f"y = y - {env.val(dy)};")
+    elif isinstance(op, ir.Edge):
+        h = op.h0 * ir.OODPPO if op.chroma else op.h0
+        t1 = f(h * ir.fn("tan", op.e))
+        psi = op.e - h * 2.0 * op.fringe / ir.fn("cos", op.e) * (ir.E(1.0) + ir.fn("sin", op.e) * ir.fn("sin", op.e))
+        t2 = f(h * ir.fn("tan", psi))
+        if not is_zero(t1):
+            out.append(f"px = {_fma('x', env.val(t1), 'px')};")
+        if not is_zero(t2):
+            out.append(f"py = {_fma('-y', env.val(t2), 'py')};")
+    elif isinstance(op, ir.ThickQuad):
+        emit_thick_quad(op, env, out)
+    elif isinstance(op, ir.ThickBend):
+        emit_thick_bend(op, env, out)
+    elif isinstance(op, ir.Wire):
+        wx, wy, k = env.val(op.wx), env.val(op.wy), env.val(op.k)
+        a, b, c = env.new_tmp(), env.new_tmp(), env.new_tmp()
+        out.append(f"{{ ufloat {a} = {wx} - x; ufloat {b} = {wy} - y; "
+                   f"ufloat {c} = {k} / {_fma(a, a, f'{b} * {b}')}; "
+                   f"px = {_fma(c, a, 'px')}; py = {_fma(c, b, 'py')}; }}")
+    elif isinstance(op, ir.Aperture):
+        # verbatim splice: the window's precedence against "== 0" is C's
+        out.append(f"if ({op.window} == 0) x = UFO_NAN;")
+    elif isinstance(op, ir.Code):
+        out.append("{ " + op.text + " }")
+    elif isinstance(op, ir.Observe):
+        if observe is None:
+            raise ValueError("observation point in a kernel without dumps")
+        out.append(observe)
+    else:
+        raise TypeError(f"no emitter for {op!r}")
+
+
+def kick_coeffs(op: ir.Kick, env: Env):
+    """Folded Horner coefficients d_n = c_n (times oodppo) / n!."""
+    n = max(len(op.kn), len(op.ks))
+    out = []
+    for i in range(n):
+        cr = op.kn[i] if i < len(op.kn) else ir.ZERO
+        ci = op.ks[i] if i < len(op.ks) else ir.ZERO
+        if op.chroma:
+            cr = cr * ir.OODPPO
+            ci = ci * ir.OODPPO
+        fact = float(math.factorial(i))
+        if i > 1:
+            cr = cr / fact
+            ci = ci / fact
+        out.append((env.f(cr), env.f(ci)))
+    return out
+
+
+def emit_kick(op: ir.Kick, env: Env, out: List[str]):
+    """w = sum d_n z^n by Horner, px -= Re w, py += Im w; the last level
+    accumulates straight into px/py."""
+    d = kick_coeffs(op, env)
+    top = -1
+    for i, (cr, ci) in enumerate(d):
+        if not is_zero(cr) or not is_zero(ci):
+            top = i
+    if top < 0:
+        return
+    if top == 0:
+        cr, ci = d[0]
+        if not is_zero(cr):
+            out.append(f"px = px - {env.val(cr)};")
+        if not is_zero(ci):
+            out.append(f"py = py + {env.val(ci)};")
+        return
+    lines = []
+    wr = None if is_zero(d[top][0]) else env.val(d[top][0])
+    wi = None if is_zero(d[top][1]) else env.val(d[top][1])
+    for lvl in range(top - 1, 0, -1):
+        dr, di = d[lvl]
+        dr = None if is_zero(dr) else env.val(dr)
+        di = None if is_zero(di) else env.val(di)
+        # new_r = wr*x - wi*y + dr ; new_i = wr*y + wi*x + di
+        r = _chain([(wr, "x", False), (wi, "y", True)], dr)
+        i = _chain([(wr, "y", False), (wi, "x", False)], di)
+        tr, ti = env.new_tmp(), env.new_tmp()
+        lines.append(f"ufloat {tr} = {r if r else env.lit(0.0)};")
+        lines.append(f"ufloat {ti} = {i if i else env.lit(0.0)};")
+        wr, wi = tr, ti
+    d0r = None if is_zero(d[0][0]) else env.val(d[0][0])
+    d0i = None if is_zero(d[0][1]) else env.val(d[0][1])
+    # px -= wr*x - wi*y + d0r ; py += wr*y + wi*x + d0i
+    px = "px"
+    if wr is not None:
+        px = _fma(_negs(wr), "x", px)
+    if wi is not None:
+        px = _fma(wi, "y", px)
+    if d0r is not None:
+        px = f"({px} - {d0r})"
+    py = "py"
+    if wr is not None:
+        py = _fma(wr, "y", py)
+    if wi is not None:
+        py = _fma(wi, "x", py)
+    if d0i is not None:
+        py = f"({py} + {d0i})"
+    if px != "px":
+        lines.append(f"px = {px};")
+    if py != "py":
+        lines.append(f"py = {py};")
+    out.append("{ " + " ".join(lines) + " }")
+
+
+def _negs(s: str) -> str:
+    return s[1:] if s.startswith("-") else f"-{s}"
+
+
+def _chain(terms, add):
+    """a1*b1 (+/-) a2*b2 + add as nested fma; None when every term is zero."""
+    acc = add
+    for coef, var, negate in reversed(terms):
+        if coef is None:
+            continue
+        c = _negs(coef) if negate else coef
+        acc = _fma(c, var, acc) if acc is not None else f"({c} * {var})"
+    return acc
+
+
+def _thick_plane(env: Env, u: str, pu: str, m11, m12, m21, m22, du=None, dpu=None):
+    """u' = m11 u + m12 pu (+ du); pu' = m21 u + m22 pu (+ dpu)."""
+    a = env.new_tmp()
+    v11, v12, v21, v22 = (env.val(m) for m in (m11, m12, m21, m22))
+    first = f"{pu} * {v12}" if du is None else _fma(pu, v12, env.val(du))
+    second = f"{pu} * {v22}" if dpu is None else _fma(pu, v22, env.val(dpu))
+    return (f"{{ ufloat {a} = {u}; {u} = {_fma(a, v11, first)}; "
+            f"{pu} = {_fma(a, v21, second)}; }}")
+
+
+
+
+def emit_thick_quad(op: ir.ThickQuad, env: Env, out: List[str]):
+    """ufo/methods.py:106-132: K = sqrt(|k1 oodppo|); focusing in x for
+    k1 > 0, else in y (k1 = 0 gives 0/0 = NaN, as in the reference)."""
+    k = op.k1
+    kk = k * ir.OODPPO if op.chroma else k
+    k2 = ir.fn("sqrt", ir.fn("fabs", kk))
+    kl = op.L * k2
+    C, S = ir.fn("cos", kl), ir.fn("sin", kl)
+    CH, SH = ir.fn("cosh", kl), ir.fn("sinh", kl)
+    foc = (C, S / k2, -(S * k2), C)
+    dfc = (CH, SH / k2, SH * k2, CH)
+    cond = ir.gt(k, ir.ZERO)
+    fx = tuple(ir.select(cond, a, b) for a, b in zip(foc, dfc))
+    fy = tuple(ir.select(cond, b, a) for a, b in zip(foc, dfc))
+    out.append(_thick_plane(env, "x", "px", *fx))
+    out.append(_thick_plane(env, "y", "py", *fy))
+
+
+def _bend_plane(env: Env, kexpr: Expr, L: Expr, disp: Optional[Expr]):
+    """Matrix (and dispersion) of one bend plane with focusing k, following
+    the three-way sign branch of ufo/methods.py:157-193."""
+    k2 = ir.fn("sqrt", ir.fn("fabs", kexpr))
+    kl = L * k2
+    C, S = ir.fn("cos", kl), ir.fn("sin", kl)
+    CH, SH = ir.fn("cosh", kl), ir.fn("sinh", kl)
+    one, zero = ir.E(1.0), ir.ZERO
+    pos = [C, S / k2, -(S * k2), C]
+    neg = [CH, SH / k2, SH * k2, CH]
+    nul = [one, L, zero, one]
+    if disp is not None:
+        pos += [disp * (one - C) / ir.fn("fabs", kexpr), disp * S / k2]
+        neg += [disp * (CH - one) / ir.fn("fabs", kexpr), disp * SH / k2]
+        nul += [zero, zero]
+    gt, lt = ir.gt(kexpr, ir.ZERO), ir.lt(kexpr, ir.ZERO)
+    return [ir.select(gt, p, ir.select(lt, n, z)) for p, n, z in zip(pos, neg, nul)]
+
+
+def emit_thick_bend(op: ir.ThickBend, env: Env, out: List[str]):
+    """ufo/methods.py:142-194: y plane with k = k1 (oodppo), x plane with
+    k + h^2 / oodppo and dispersion dp h (1-C)/|k|, dp h S/K."""
+    h = op.angle / op.L
+    k = op.k1
+    if op.chroma:
+        h = h * ir.OODPPO
+        k = k * ir.OODPPO
+        kx = k + h * h / ir.OODPPO
+    else:
+        kx = k + h * h
+    my = _bend_plane(env, k, op.L, None)
+    mx = _bend_plane(env, kx, op.L, ir.DP * h)
+    dx = None if is_zero(env.f(mx[4])) else mx[4]
+    dpx = None if is_zero(env.f(mx[5])) else mx[5]
+    out.append(_thick_plane(env, "x", "px", mx[0], mx[1], mx[2], mx[3], dx, dpx))
+    out.append(_thick_plane(env, "y", "py", *my))
+
+
+# ---------------------------------------------------------------------------
+# Turn structure: period detection
+# ---------------------------------------------------------------------------
+
+def find_period(ops):
+    """Largest coverage (head, period, repeats) of the op list by a repeated
+    block starting within the first few ops; (0, 0, 0) when none repeats."""
+    n = len(ops)
+    if n < 2:
+        return 0, 0, 0
+    ids = {}
+    seq = np.array([ids.setdefault(op, len(ids)) for op in ops], dtype=np.int64)
+    best = (0, 0, 0)
+    best_cov = 0
+    for head in range(0, min(9, n - 1)):
+        s = seq[head:]
+        m = len(s)
+        for p in range(1, m // 2 + 1):
+            eq = s[p:] == s[:-p]
+            bad = np.flatnonzero(~eq)
+            run = (len(eq) if len(bad) == 0 else bad[0]) + p
+            reps = run // p
+            if reps >= 2 and reps * p > best_cov:
+                best_cov = reps * p
+                best = (head, p, reps)
+        if best_cov == m:
+            break
+    return best
+
+
+# ---------------------------------------------------------------------------
+# Kernel assembly
+# ---------------------------------------------------------------------------
+
+_PRELUDE = """// generated by ufo_amd.codegen — do not edit
+typedef {ufloat} ufloat;
+#define UFO_NAN {nan}
+#ifndef NAN
+#define NAN UFO_NAN
+#endif
+"""
+
+
+@dataclass
+class Kernel:
+    """Generated source plus what the driver needs to launch it."""
+    name: str
+    src: str
+    kind: str
+    fp32: bool
+    flops_per_turn: int        # canonical FLOPs per particle-turn
+    elements: int              # flattened elements excluding markers/apertures
+    dumps_per_turn: int = 0
+    block: int = 256
+    period: tuple = (0, 0, 0)
+
+
+def _layout(body_ops, env, observe=None, loop_var="rep"):
+    """Emit the op list, as a loop over the repeated period when there is
+    one."""
+    head, p, reps = find_period(body_ops) if len(body_ops) > 8 else (0, 0, 0)
+    out: List[str] = []
+    if reps >= 2:
+        for op in body_ops[:head]:
+            emit_op(op, env, out, observe)
+        out.append("#pragma clang loop unroll(disable)")
+        out.append(f"for (int {loop_var} = 0; {loop_var} < {reps}; ++{loop_var}) {{")
+        inner: List[str] = []
+        for op in body_ops[head:head + p]:
+            emit_op(op, env, inner, observe)
+        out += ["    " + s for s in inner]
+        out.append("}")
+        for op in body_ops[head + p * reps:]:
+            emit_op(op, env, out, observe)
+    else:
+        for op in body_ops:
+            emit_op(op, env, out, observe)
+    return out, (head, p, reps)
+
+
+def _coord_sources(parameters, flags, dp):
+    """Initial value of each coordinate (ufo/track.py:93-101): a parameter
+    column, or 0; FIVED forces z = 0 and dp = the literal."""
+    src = {}
+    for c in COORDS:
+        src[c] = ("p", list(parameters).index(c)) if c in parameters else ("c", 0.0)
+    if flags & FIVED:
+        src["z"] = ("c", 0.0)
+        src["dp"] = ("c", float(dp))
+    return src
+
+
+def _env_for(flags, src):
+    fp32 = not (flags & DOUBLE_PRECISION)
+    consts = {}
+    if src["dp"][0] == "c":
+        dpv = src["dp"][1]
+        consts["dp"] = dpv
+        if fp32:
+            one = np.float32(1.0)
+            consts["oodppo"] = float(one / (np.float32(dpv) + one))
+        else:
+            consts["oodppo"] = 1.0 / (dpv + 1.0)
+    return Env(fp32=fp32, consts=consts)
+
+
+def _load_block(env: Env, src, P: int, row: str, indent: str):
+    """Statements that load one particle: coordinates, parameters used by the
+    body, then the hoisted invariants (which may use both)."""
+    lines = []
+    for c in COORDS:
+        kind, v = src[c]
+        if kind == "p":
+            lines.append(f"{c} = {row}[{v}];")
+        else:
+            lines.append(f"{c} = {env.lit(v)};")
+    if "oodppo" in env.consts:
+        lines.append(f"oodppo = {env.lit(env.consts['oodppo'])};")
+    else:
+        one = env.lit(1.0)
+        lines.append(f"oodppo = {one} / (dp + {one});")
+    for i in sorted(env.params_used):
+        if i >= P:
+            raise IndexError(f"parameters[{i}] used but only {P} parameter columns exist")
+        lines.append(f"P{i} = {row}[{i}];")
+    lines += env.inv_code
+    return [indent + s for s in lines]
+
+
+def _decls(env: Env):
+    names = ["x", "px", "y", "py", "z", "dp", "oodppo"]
+    names += [f"P{i}" for i in sorted(env.params_used)]
+    names += list(env.invariants.values())
+    return "ufloat " + ", ".join(names) + ";"
+
+
+def _count(ops, env):
+    flops = sum(ir.op_flops(op.map_exprs(env.f)) for op in ops)
+    return flops
+
+
+def element_count(line) -> int:
+    from .elements import Marker, Aperture
+    return sum(1 for e in line.flatten() if not isinstance(e, (Marker, Aperture)))
+
+
+def sa_kernel(line, flags, turns, parameters, dp, block=256, name="ufo_sa") -> Kernel:
+    """StableAperture: lost[p] = number of turns completed before the
+    end-of-turn test |x| > 1 || |y| > 1 || isnan(x) fired, or ``turns``
+    (ufo/sa.py:116-121)."""
+    pieces = line.method(flags=flags, parameters=parameters)
+    ops = [op for piece in pieces for op in piece]
+    src = _coord_sources(parameters, flags, dp)
+    env = _env_for(flags, src)
+    body, period = _layout(ops, env)
+    P = max(1, len(parameters))
+    one = env.lit(1.0)
+    load = _load_block(env, src, P, "row", "            ")
+    code = _PRELUDE.format(ufloat="float" if env.fp32 else "double",
+                           nan='__builtin_nanf("")' if env.fp32 else '__builtin_nan("")')
+    code += f"""
+extern "C" __global__ void __launch_bounds__({block})
+{name}(const ufloat *__restrict__ pool, unsigned int *__restrict__ lost,
+       int *__restrict__ counter, int particles, int first)
+{{
+    // lanes start on particles [0, first); further particles are claimed
+    // from counter (zeroed by the host before the launch)
+    {_decls(env)}
+    int idx = blockIdx.x * {block} + threadIdx.x;
+    int turn = 0;
+    bool need = true;
+    for (;;) {{
+        if (need) {{
+            if (idx >= particles) break;
+            const ufloat *__restrict__ row = pool + (size_t)idx * {P};
+{chr(10).join(load)}
+            turn = 0;
+            need = false;
+        }}
+{chr(10).join('        ' + s for s in body)}
+        bool gone = fabs(x) > {one} || fabs(y) > {one} || isnan(x);
+        bool done = gone || turn + 1 >= {turns};
+        if (done) {{
+            lost[idx] = gone ? turn : {turns};
+            unsigned long long m = __ballot(1);
+            int lane = __lane_id();
+            int leader = __ffsll((long long)m) - 1;
+            int rank = __popcll(m & ((1ull << lane) - 1ull));
+            int base = 0;
+            if (lane == leader) base = atomicAdd(counter, __popcll(m));
+            base = __shfl(base, leader);
+            idx = first + base + rank;
+            need = true;
+        }} else {{
+            ++turn;
+        }}
+    }}
+}}
+"""
+    return Kernel(name=name, src=code, kind="sa", fp32=env.fp32, flops_per_turn=_count(ops, env),
+                  elements=element_count(line), block=block, period=period)
+
+
+def track_kernel(line, flags, turns, parameters, where, dp, block=256, name="ufo_track") -> Kernel:
+    """Track: samples at the fracture points of ``where`` (and at the end of
+    the line for -1), in lattice order, turn after turn; a particle stops
+    after the turn at whose end x is NaN (ufo/track.py:62-114).  Output is
+    [sample][coordinate][particle]."""
+    where = list(where) if isinstance(where, (list, tuple)) else [where]
+    pieces = line.method(fracture=where, flags=flags, parameters=parameters)
+    ops = []
+    for i, piece in enumerate(pieces):
+        if i:
+            ops.append(ir.Observe())
+        ops += piece
+    if -1 in where:
+        ops.append(ir.Observe())
+    dumps = sum(1 for op in ops if isinstance(op, ir.Observe))
+    src = _coord_sources(parameters, flags, dp)
+    env = _env_for(flags, src)
+    store = ("{ size_t b = (size_t)s * 6 * np + idx; "
+             "tracks[b] = x; tracks[b + np] = px; tracks[b + 2 * np] = y; "
+             "tracks[b + 3 * np] = py; tracks[b + 4 * np] = z; tracks[b + 5 * np] = dp; ++s; }")
+    body, period = _layout(ops, env, observe=store)
+    P = max(1, len(parameters))
+    load = _load_block(env, src, P, "row", "    ")
+    code = _PRELUDE.format(ufloat="float" if env.fp32 else "double",
+                           nan='__builtin_nanf("")' if env.fp32 else '__builtin_nan("")')
+    code += f"""
+extern "C" __global__ void __launch_bounds__({block})
+{name}(const ufloat *__restrict__ pool, ufloat *__restrict__ tracks, int particles)
+{{
+    {_decls(env)}
+    const int idx = blockIdx.x * {block} + threadIdx.x;
+    if (idx >= particles) return;
+    const size_t np = (size_t)particles;
+    const ufloat *__restrict__ row = pool + (size_t)idx * {P};
+{chr(10).join(load)}
+    int s = 0;
+    for (int turn = 0; turn < {turns}; ++turn) {{
+{chr(10).join('        ' + s_ for s_ in body)}
+        if (isnan(x)) break;
+    }}
+}}
+"""
+    return Kernel(name=name, src=code, kind="track", fp32=env.fp32, flops_per_turn=_count(ops, env),
+                  elements=element_count(line), dumps_per_turn=dumps, block=block, period=period)
+
+
+def optics_kernel(line, flags, where, parameters, name="ufo_optics") -> Kernel:
+    """Optics: three rays from pool row 0 — x=y=1, px=py=1, dp=1 (plus a
+    'dp' parameter if named) — dumped at the observation points
+    (ufo/optics.py:87-115).  Output [3][count][6]; only x, px, y, py are
+    written."""
+    pieces = line.method(fracture=where, flags=flags, parameters=parameters)
+    ops = []
+    for i, piece in enumerate(pieces):
+        if i:
+            ops.append(ir.Observe())
+        ops += piece
+    if -1 in where:
+        ops.append(ir.Observe())
+    dumps = sum(1 for op in ops if isinstance(op, ir.Observe))
+    fp32 = not (flags & DOUBLE_PRECISION)
+    env = Env(fp32=fp32, consts={})
+    store = ("{ ufloat *o = tracks + ((size_t)ray * count + s) * 6; "
+             "o[0] = x; o[1] = px; o[2] = y; o[3] = py; ++s; }")
+    body, _ = _layout(ops, env, observe=store)
+    P = max(1, len(parameters))
+    zero, one = env.lit(0.0), env.lit(1.0)
+    lines = [f"x = y = z = px = py = dp = {zero};",
+             f"if (ray == 0) {{ x = {one}; y = {one}; }}",
+             f"if (ray == 1) {{ px = {one}; py = {one}; }}",
+             f"if (ray == 2) dp = {one};"]
+    if "dp" in parameters:
+        lines.append(f"dp += row[{list(parameters).index('dp')}];")
+    lines.append(f"oodppo = {one} / (dp + {one});")
+    for i in sorted(env.params_used):
+        lines.append(f"P{i} = row[{i}];")
+    lines += env.inv_code
+    code = _PRELUDE.format(ufloat="float" if fp32 else "double",
+                           nan='__builtin_nanf("")' if fp32 else '__builtin_nan("")')
+    code += f"""
+extern "C" __global__ void __launch_bounds__(64)
+{name}(const ufloat *__restrict__ pool, ufloat *__restrict__ tracks, int count)
+{{
+    {_decls(env)}
+    const int ray = threadIdx.x;
+    if (ray >= 3) return;
+    const ufloat *__restrict__ row = pool;
+{chr(10).join('    ' + s_ for s_ in lines)}
+    int s = 0;
+{chr(10).join('    ' + s_ for s_ in body)}
+    (void)s;
+}}
+"""
+    return Kernel(name=name, src=code, kind="optics", fp32=fp32, flops_per_turn=_count(ops, env),
+                  elements=element_count(line), dumps_per_turn=dumps, block=64)
+
+
+HIPRTC_OPTIONS = ("-O3 -std=c++17 -ffp-contract=off -Xclang -target-feature -Xclang -packed-fp32-ops "
+                  "-Wno-bool-compare -Wno-unused-variable -Wno-unused-but-set-variable")

@@ -1,0 +1,9 @@
+"""Physical constants (CODATA values; the reference keeps the same names in
+ufo/constants.py:9-14)."""
+
+speed_of_light = 299792458.0              # m/s
+electron_mass = 0.51099895000e6           # eV
+proton_mass = 938.27208816e6              # eV
+elementary_charge = 1.602176634e-19       # C
+vacuum_permittivity = 8.8541878128e-12    # F/m
+classical_electron_radius = 2.8179403262e-15  # m

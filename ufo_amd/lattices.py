@@ -1,0 +1,104 @@
+"""Synthetic lattices of the tracking workloads (FP64 teapot tracking).
+
+``fodo_ring(cells)``
+    cells x [QF L=0.5 m k1=+1.2 m^-2 (4 slices), SF thin sextupole
+    k2L=+0.8 m^-2, drift 2.0 m, QD L=0.5 m k1=-1.2 m^-2 (4 slices), SD thin
+    sextupole k2L=-1.4 m^-2, drift 2.0 m], then an aperture |x|,|y| < 40 mm
+    checked once per turn.  The thin sextupoles are Multipoles: a zero-length
+    Sextupole would kick with zero strength (the teapot scales by L/n).
+
+``dba_ring(periods)``
+    periods x a 16.9 m double-bend period of 25 elements: 2 combined-function
+    sector bends (L=1.384 m, angle 2 pi/32, k1=-0.4 m^-2, 8 slices), 6
+    quadrupoles (L 0.25-0.5 m, |k1| 1.0-2.2 m^-2, 8 slices), 4 sextupoles
+    (L=0.2 m, k2 = +-40 m^-3, 4 slices) and 13 drifts; an aperture
+    |x|,|y| < 30 mm at the end of the ring.  Per period the linear tunes are
+    0.604 (x) and 0.243 (y), so 16 periods sit at Qx = 9.664, on the
+    third-integer resonance driven by the sextupoles: the dynamic-aperture
+    boundary enters the corner of the x in [-20, 20] mm, y in [0, 8] mm scan.
+"""
+from __future__ import annotations
+
+import math
+
+from .elements import Aperture, Drift, Multipole, Quadrupole, Sbend, Sextupole
+from .lattice import Line
+
+
+def fodo_cell(tag=""):
+    qf = Quadrupole(f"QF{tag}", length=0.5, k1=1.2, slices=4)
+    qd = Quadrupole(f"QD{tag}", length=0.5, k1=-1.2, slices=4)
+    sf = Multipole(f"SF{tag}", knl=[0.0, 0.0, 0.8])
+    sd = Multipole(f"SD{tag}", knl=[0.0, 0.0, -1.4])
+    d = Drift(f"D{tag}", length=2.0)
+    return Line(f"CELL{tag}", [qf, sf, d, qd, sd, d])
+
+
+def fodo_ring(cells=16, aperture=0.04, distinct=False):
+    """FODO ring; ``distinct`` gives every cell its own element objects and
+    labels (QF0, QF1, ...) so each magnet can be addressed separately."""
+    if distinct:
+        body = [fodo_cell(str(i)) for i in range(cells)]
+    else:
+        cell = fodo_cell()
+        body = [cell] * cells
+    ap = Aperture("AP", window=f"(fabs(x) < {aperture} && fabs(y) < {aperture})")
+    return Line("RING", body + [ap])
+
+
+DBA_QUADS = {"QF1": (0.25, 2.2), "QD1": (0.30, -1.0), "QF2": (0.50, 1.5),
+             "QF3": (0.50, 1.5), "QD2": (0.30, -1.0), "QF4": (0.25, 2.2)}
+DBA_SEXTS = {"SD1": -40.0, "SF1": 40.0, "SF2": 40.0, "SD2": -40.0}
+
+
+def dba_period(tag=""):
+    """One 16.9 m period (mirror-symmetric layout, distinct family labels)."""
+    ang = 2.0 * math.pi / 32.0
+
+    def q(name):
+        L, k1 = DBA_QUADS[name]
+        return Quadrupole(f"{name}{tag}", length=L, k1=k1, slices=8)
+
+    def s(name):
+        return Sextupole(f"{name}{tag}", length=0.2, k2=DBA_SEXTS[name], slices=4)
+
+    def b(name):
+        return Sbend(f"{name}{tag}", length=1.384, angle=ang, k1=-0.4, slices=8)
+
+    def d(name, L):
+        return Drift(f"{name}{tag}", length=L)
+
+    els = [d("DSS", 4.066), q("QF1"), d("D1", 0.25), q("QD1"), d("D2", 0.35), b("B1"),
+           d("D3", 0.30), s("SD1"), d("D4", 0.20), q("QF2"), d("D5", 0.20), s("SF1"), d("DC", 0.50),
+           s("SF2"), d("D5B", 0.20), q("QF3"), d("D4B", 0.20), s("SD2"), d("D3B", 0.30), b("B2"),
+           d("D2B", 0.35), q("QD2"), d("D1B", 0.25), q("QF4"), d("DSSB", 4.066)]
+    return Line(f"PERIOD{tag}", els)
+
+
+def dba_ring(periods=16, aperture=0.03, distinct=False):
+    if distinct:
+        body = [dba_period(str(i)) for i in range(periods)]
+    else:
+        p = dba_period()
+        body = [p] * periods
+    ap = Aperture("AP", window=f"(fabs(x) < {aperture} && fabs(y) < {aperture})")
+    return Line("RING", body + [ap])
+
+
+def da_grid(nx, ny, xlim=0.02, ymax=0.008):
+    """Initial (x, y) of an nx x ny dynamic-aperture grid, x fastest."""
+    import numpy as np
+    x = np.linspace(-xlim, xlim, nx)
+    y = np.linspace(0.0, ymax, ny)
+    xx, yy = np.meshgrid(x, y)
+    return np.stack([xx.ravel(), yy.ravel()], axis=1)
+
+
+def gaussian_bunch(n, seed=0, sx=1e-3, spx=1e-4, sy=1e-3, spy=1e-4, sdp=1e-3):
+    """Gaussian bunch [n, 5] columns x, px, y, py, dp."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    out = np.empty((n, 5))
+    for j, s in enumerate((sx, spx, sy, spy, sdp)):
+        out[:, j] = rng.normal(0.0, s, n)
+    return out

@@ -1,0 +1,91 @@
+"""StableAperture: turns survived by each particle (ufo/sa.py:18-145)."""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from . import codegen
+from .sim import Simulation, host_buffer
+
+
+class StableAperture(Simulation):
+    """Track each particle until it is lost or ``turns`` turns are done.
+
+    Same arguments and attributes as the reference (ufo/sa.py:37-145):
+    ``parameters`` is [particles, max(1, len(parameters))]; after ``run()``
+    ``lost[p]`` is the number of turns completed before the end-of-turn test
+    ``|x| > 1 or |y| > 1 or isnan(x)`` fired, or ``turns`` if the particle
+    survived.  ``threads`` is accepted for compatibility; lanes are
+    persistent and the launch is sized to fill the device.
+    """
+
+    def __init__(self, line, flags=0x00, turns=1000, particles=1000, parameters=(), dp=0.,
+                 context=None, options=None):
+        self._init_common(flags, context, options)
+        self.turns = int(turns)
+        self.particles = int(particles)
+        self.parameter_names = list(parameters)
+        self.kernel = codegen.sa_kernel(line, flags, self.turns, self.parameter_names, dp)
+        P = max(1, len(self.parameter_names))
+        self._pool_host = host_buffer((self.particles, P), self.dtype)
+        self.parameters = self._pool_host.numpy()
+        self._lost_host = host_buffer((self.particles,), torch.int32)
+        self.lost = self._lost_host.numpy().view(np.uint32)
+        self._pool_dev = None
+        self._lost_dev = None
+        self._counter = None
+        self.lanes = None
+
+    def _buffers(self):
+        if self._pool_dev is None:
+            self._pool_dev = torch.empty(self._pool_host.shape, dtype=self.dtype, device=self.device)
+            self._lost_dev = torch.empty((self.particles,), dtype=torch.int32, device=self.device)
+            self._counter = torch.zeros((1,), dtype=torch.int32, device=self.device)
+
+    @property
+    def pool_device(self):
+        self._buffers()
+        return self._pool_dev
+
+    @property
+    def lost_device(self):
+        self._buffers()
+        return self._lost_dev
+
+    def launch_config(self, lanes=None):
+        block = self.kernel.block
+        if lanes is None:
+            lanes = self.resident_lanes()
+        lanes = max(1, min(lanes, self.particles))
+        grid = (lanes + block - 1) // block
+        return grid, block
+
+    def launch(self, lanes=None):
+        """Enqueue one pass on the current stream with the device pool as
+        input (no host copies)."""
+        self._buffers()
+        if self.particles == 0:
+            return
+        if self.turns <= 0:
+            self._lost_dev.zero_()
+            return
+        grid, block = self.launch_config(lanes)
+        self.lanes = grid * block
+        self._counter.zero_()
+        self.module().launch(self.kernel.name, grid, block,
+                             [self._pool_dev, self._lost_dev, self._counter, self.particles, grid * block])
+
+    def run(self, threads=1):
+        """Copy ``parameters`` to the device, track, copy ``lost`` back."""
+        self._buffers()
+        self._pool_dev.copy_(self._pool_host, non_blocking=True)
+        self.launch()
+        self._lost_host.copy_(self._lost_dev, non_blocking=True)
+        self.stream().synchronize()
+
+    def executed_turns(self, lost=None):
+        """Turns executed per particle: lost+1 for lost particles, turns for
+        survivors."""
+        lost = self.lost if lost is None else lost
+        lost = np.asarray(lost, dtype=np.int64)
+        return np.minimum(lost + 1, self.turns)

@@ -1,0 +1,82 @@
+"""Track: tracking with the particle coordinates sampled at observation
+points every turn (ufo/track.py:19-139)."""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from . import codegen
+from .sim import Simulation, host_buffer
+
+
+class Track(Simulation):
+    """Track a bunch through ``line`` for ``turns`` turns, recording
+    (x, px, y, py, z, dp) at the positions ``where`` (element index plus
+    fraction of that element; -1 = end of line) every turn.
+
+    Same arguments and attributes as the reference (ufo/track.py:40-139):
+    ``parameters`` is a host array [particles, len(parameters)] the user fills
+    before ``run()``; ``tracks`` is [particles, len(where)*turns, 6], samples
+    in lattice order, turn after turn.  Samples after a particle is lost stay
+    NaN (the reference leaves them unwritten); under FIVED, z and dp hold 0
+    and the ``dp`` argument.  ``run()`` blocks until ``tracks`` is filled.
+    ``threads`` is accepted for compatibility; the launch is sized from the
+    particle count (one lane per particle).
+    """
+
+    def __init__(self, line, flags=0x00, turns=1000, particles=1000, parameters=(), where=(), dp=0.,
+                 context=None, options=None):
+        self._init_common(flags, context, options)
+        where = list(where) if isinstance(where, (list, tuple)) else [where]
+        self.turns = int(turns)
+        self.particles = int(particles)
+        self.parameter_names = list(parameters)
+        self.count = len(where)
+        total = self.count * self.turns
+        if total < 1:
+            raise Exception("No valid observation points found in 'where'")
+        self.kernel = codegen.track_kernel(line, flags, self.turns, self.parameter_names, where, dp)
+        if self.kernel.dumps_per_turn > self.count:
+            raise Exception("more observation points than entries in 'where'")
+        flat = line.flatten()
+        self.where = [(idx, flat[int(idx)]) for idx in where]
+        self.samples = total
+        P = max(1, len(self.parameter_names))
+        self._pool_host = host_buffer((self.particles, P), self.dtype)
+        self.parameters = self._pool_host.numpy()
+        self._tracks_host = host_buffer((total, 6, self.particles), self.dtype)
+        self.tracks = self._tracks_host.permute(2, 0, 1).numpy()
+        self._pool_dev = None
+        self._tracks_dev = None
+
+    def _buffers(self):
+        if self._pool_dev is None:
+            self._pool_dev = torch.empty(self._pool_host.shape, dtype=self.dtype, device=self.device)
+            self._tracks_dev = torch.empty(self._tracks_host.shape, dtype=self.dtype, device=self.device)
+
+    @property
+    def tracks_device(self):
+        """Device tensor [particles, samples, 6] (a strided view of the
+        [sample][coordinate][particle] buffer) after ``run()``."""
+        self._buffers()
+        return self._tracks_dev.permute(2, 0, 1)
+
+    def launch(self):
+        """Enqueue one tracking pass on the current stream with the device
+        pool as input (no host copies)."""
+        self._buffers()
+        block = self.kernel.block
+        grid = (self.particles + block - 1) // block
+        self._tracks_dev.fill_(float("nan"))
+        if self.particles == 0:
+            return
+        self.module().launch(self.kernel.name, grid, block,
+                             [self._pool_dev, self._tracks_dev, self.particles])
+
+    def run(self, threads=1):
+        """Copy ``parameters`` to the device, track, copy ``tracks`` back."""
+        self._buffers()
+        self._pool_dev.copy_(self._pool_host, non_blocking=True)
+        self.launch()
+        self._tracks_host.copy_(self._tracks_dev, non_blocking=True)
+        self.stream().synchronize()
