@@ -1,0 +1,72 @@
+"""Test helper: the emitted Track kernel compiled for the host.
+
+The HIP source that ``ufo_amd.codegen.track_kernel`` emits is plain C++
+arithmetic (``fma``, ``+``, ``*``, comparisons); compiled by g++ with
+``-ffp-contract=off`` it performs the same IEEE operations in the same order
+as the GPU, so on lattices without transcendental functions (teapot KICK
+paths) it reproduces the GPU result bit for bit.  The GPU tests use it to
+check that the device executes exactly what was emitted; the parity bar
+itself is against the oracle.  Never used by the product.
+"""
+import ctypes
+import hashlib
+import os
+import subprocess
+import tempfile
+
+import numpy as np
+
+_PRELUDE = r"""
+#include <cmath>
+#include <cstddef>
+using std::fma; using std::sqrt; using std::fabs; using std::sin; using std::cos; using std::tan;
+using std::sinh; using std::cosh; using std::isnan;
+struct ufo_dim3 { unsigned x, y, z; };
+static ufo_dim3 blockIdx, threadIdx;
+#define __global__
+#define __launch_bounds__(n)
+"""
+
+_DRIVER = r"""
+extern "C" void host_run(const ufloat *pool, ufloat *tracks, int particles, int block)
+{
+    int blocks = (particles + block - 1) / block;
+    for (int b = 0; b < blocks; ++b) {
+        blockIdx.x = b;
+        for (int t = 0; t < block; ++t) {
+            threadIdx.x = t;
+            %s(pool, tracks, particles);
+        }
+    }
+}
+"""
+
+_CACHE = os.path.join(tempfile.gettempdir(), f"ufo_host_emulation_{os.getuid()}")
+
+
+def compile_track(kernel):
+    src = _PRELUDE + kernel.src + (_DRIVER % kernel.name)
+    key = hashlib.sha1(src.encode()).hexdigest()[:20]
+    os.makedirs(_CACHE, exist_ok=True)
+    so = os.path.join(_CACHE, f"t_{key}.so")
+    if not os.path.exists(so):
+        cpp = os.path.join(_CACHE, f"t_{key}.cpp")
+        with open(cpp, "w") as f:
+            f.write(src)
+        tmp = so + f".{os.getpid()}"
+        subprocess.run(["g++", "-std=c++17", "-O2", "-ffp-contract=off", "-fno-fast-math", "-fPIC", "-shared",
+                        "-w", "-o", tmp, cpp], check=True)
+        os.replace(tmp, so)
+    return ctypes.CDLL(so)
+
+
+def run_track(kernel, pool, samples):
+    """Host run of a generated Track kernel: returns [particles, samples, 6]."""
+    lib = compile_track(kernel)
+    dt = np.float32 if kernel.fp32 else np.float64
+    pool = np.ascontiguousarray(pool, dtype=dt)
+    n = pool.shape[0]
+    out = np.full((samples, 6, n), np.nan, dtype=dt)
+    lib.host_run(pool.ctypes.data_as(ctypes.c_void_p), out.ctypes.data_as(ctypes.c_void_p),
+                 ctypes.c_int(n), ctypes.c_int(kernel.block))
+    return out.transpose(2, 0, 1)

@@ -1,20 +1,34 @@
 """Parity helpers shared by the GPU tests.
 
-Bar: relative coordinate error ``|a - b| / max(|b|, rms_c)`` at most 1e-12
-on regular surviving particles, identical lost/alive flags.  A particle is
-regular when the oracle's strict-IEEE and fast-math builds (the reference's
-own noise floor: its kernels are compiled with -cl-fast-relaxed-math
--cl-mad-enable) agree to 1e-13 on it.
+The checker is the oracle's strict-IEEE build of the restated reference
+kernels.  Rounding noise is amplified by the dynamics, so the bar is set per
+particle from its measured sensitivity:
+
+  sensitivity  max of two differences against the strict oracle, both of
+               them the reference's own arithmetic: its fast-math build
+               (stand-in for -cl-fast-relaxed-math -cl-mad-enable) and a
+               strict run from the initial x moved by one ulp.
+  regular      same lost/alive flags in those runs and sensitivity <= 3e-13:
+               rounding is not amplified beyond 3e-13 over the run.
+
+Bar: on regular particles the HIP result is within 1e-12 (relative error
+``|a - b| / max(|b|, rms_c)``) of the strict oracle with identical
+lost/alive flags; over all particles with stable flags, the median and 90th
+percentile of its error are no larger than twice those of the reference's
+own sensitivity (plus 1e-14), i.e. HIP is as close to the reference as the
+reference is to itself.
 """
 import numpy as np
 
 REL_TOL = 1e-12
-REGULAR_TOL = 1e-13
+REGULAR_TOL = 3e-13
+STAT_FACTOR = 2.0
+STAT_FLOOR = 1e-14
 
 
 def rel_err(a, b, axis_rms=0):
     """Per-entry relative error with the rms of b along ``axis_rms`` as the
-    floor of the denominator (finite entries only)."""
+    floor of the denominator (finite entries of b only)."""
     a = np.asarray(a, dtype=np.float64)
     b = np.asarray(b, dtype=np.float64)
     fin = np.isfinite(b)
@@ -26,19 +40,54 @@ def rel_err(a, b, axis_rms=0):
     return np.where(fin, err, 0.0)
 
 
-def track_regular(strict, fast, tol=REGULAR_TOL):
-    """Particles of a Track result [particles, samples, 6] whose strict and
-    fast oracle runs agree to ``tol`` at every sample."""
-    err = rel_err(strict[:, :, :4], fast[:, :, :4])
-    same_nan = np.all(np.isnan(strict[:, :, 0]) == np.isnan(fast[:, :, 0]), axis=1)
-    return same_nan & (np.max(err, axis=(1, 2)) <= tol)
+def particle_err(a, b):
+    """Worst relative error per particle over samples and x, px, y, py."""
+    return np.max(rel_err(a[:, :, :4], b[:, :, :4]), axis=(1, 2))
 
 
-def compare_tracks(gpu, strict, fast):
-    """Returns (max rel err on regular particles, regular mask, nan flags
-    equal on regular particles)."""
-    reg = track_regular(strict, fast)
-    err = rel_err(gpu[:, :, :4], strict[:, :, :4])
-    flags_ok = np.array_equal(np.isnan(gpu[reg, :, 0]), np.isnan(strict[reg, :, 0]))
-    worst = float(np.max(err[reg])) if np.any(reg) else 0.0
-    return worst, reg, flags_ok
+def nan_flags(t):
+    return np.isnan(np.asarray(t, dtype=np.float64)[:, :, 0])
+
+
+def reference_runs(track_fn, pool):
+    """strict, fast and one-ulp-perturbed strict runs of the oracle;
+    ``track_fn(pool, mode)`` runs it."""
+    strict = track_fn(pool, "strict")
+    fast = track_fn(pool, "fast")
+    moved = np.array(pool, dtype=np.float64, copy=True)
+    moved[:, 0] = np.nextafter(moved[:, 0], np.inf)
+    perturbed = track_fn(moved, "strict")
+    return strict, fast, perturbed
+
+
+def compare_tracks(gpu, strict, fast, perturbed):
+    ns = nan_flags(strict)
+    stable = np.all(ns == nan_flags(fast), axis=1) & np.all(ns == nan_flags(perturbed), axis=1)
+    sens = np.maximum(particle_err(fast, strict), particle_err(perturbed, strict))
+    e_gpu = particle_err(gpu, strict)
+    regular = stable & (sens <= REGULAR_TOL)
+    flags_ok = np.array_equal(nan_flags(gpu)[stable], ns[stable])
+    r = {
+        "regular": regular,
+        "flags_ok": flags_ok,
+        "worst_regular": float(np.max(e_gpu[regular])) if np.any(regular) else 0.0,
+        "median_gpu": float(np.median(e_gpu[stable])) if np.any(stable) else 0.0,
+        "median_ref": float(np.median(sens[stable])) if np.any(stable) else 0.0,
+        "p90_gpu": float(np.percentile(e_gpu[stable], 90)) if np.any(stable) else 0.0,
+        "p90_ref": float(np.percentile(sens[stable], 90)) if np.any(stable) else 0.0,
+    }
+    return r
+
+
+def summary(r):
+    return (f"regular {int(r['regular'].sum())}/{len(r['regular'])}, worst on regular {r['worst_regular']:.2e}; "
+            f"median err hip {r['median_gpu']:.2e} vs reference self {r['median_ref']:.2e}; "
+            f"p90 hip {r['p90_gpu']:.2e} vs {r['p90_ref']:.2e}")
+
+
+def assert_parity(r, min_regular_frac=0.25):
+    assert r["flags_ok"], "lost/alive flags differ on particles whose flags are stable"
+    assert r["worst_regular"] <= REL_TOL, summary(r)
+    assert r["median_gpu"] <= STAT_FACTOR * r["median_ref"] + STAT_FLOOR, summary(r)
+    assert r["p90_gpu"] <= STAT_FACTOR * r["p90_ref"] + STAT_FLOOR, summary(r)
+    assert r["regular"].mean() >= min_regular_frac, summary(r)

@@ -15,7 +15,8 @@ from ufo_amd import lattices
 from oracle import runner
 
 from conftest import OPTICS_DIR
-from parity import REL_TOL, compare_tracks, rel_err
+import host_emulation
+from parity import assert_parity, compare_tracks, reference_runs, summary
 
 pytestmark = pytest.mark.gpu
 
@@ -23,14 +24,28 @@ COORD6 = ["x", "px", "y", "py", "dp"]
 
 
 def _track_both(line, pool, flags, turns, parameters, where, dp=0.0):
+    """HIP Track result and the three oracle runs (strict, fast-math, strict
+    from a one-ulp-moved x).  On teapot lattices the HIP result must also
+    equal the host build of the emitted kernel bit for bit."""
     tr = ufo.Track(line, flags=flags, turns=turns, particles=len(pool), parameters=parameters, where=where, dp=dp)
     tr.parameters[:] = pool
     tr.run()
     gpu = np.array(tr.tracks)
-    strict = runner.track(line, pool, flags=flags, turns=turns, parameters=parameters, where=where, dp=dp)
-    fast = runner.track(line, pool, flags=flags, turns=turns, parameters=parameters, where=where, dp=dp,
-                        mode="fast")
-    return gpu, strict, fast
+
+    def oracle(p, mode):
+        return runner.track(line, p, flags=flags, turns=turns, parameters=parameters, where=where, dp=dp,
+                            mode=mode)
+
+    refs = reference_runs(oracle, pool)
+    if not _has_transcendentals(tr.kernel.src):
+        host = host_emulation.run_track(tr.kernel, tr.parameters, tr.samples)
+        same = (gpu == host) | (np.isnan(gpu) & np.isnan(host))
+        assert same.all(), f"HIP differs from the host build of the emitted kernel in {np.sum(~same)} entries"
+    return (gpu,) + refs
+
+
+def _has_transcendentals(src):
+    return any(f"{f}(" in src for f in ("sqrt", "sin", "cos", "tan", "sinh", "cosh")) or " / " in src
 
 
 def test_fodo_track_6d():
@@ -38,13 +53,11 @@ def test_fodo_track_6d():
     ring = lattices.fodo_ring(16)
     bunch = lattices.gaussian_bunch(512, seed=0)
     bunch[:8, 0] = np.linspace(0.03, 0.06, 8)      # some lost at the aperture
-    gpu, strict, fast = _track_both(ring, bunch, ufo.KICK | ufo.DOUBLE_PRECISION, 200, COORD6, [-1])
-    worst, reg, flags_ok = compare_tracks(gpu, strict, fast)
-    print(f"regular {reg.sum()}/{len(reg)}, worst rel err {worst:.3e}")
-    assert flags_ok
+    gpu, strict, fast, moved = _track_both(ring, bunch, ufo.KICK | ufo.DOUBLE_PRECISION, 100, COORD6, [-1])
+    r = compare_tracks(gpu, strict, fast, moved)
+    print(summary(r))
+    assert_parity(r, 0.3)
     assert np.array_equal(np.isnan(gpu[:, :, 0]), np.isnan(strict[:, :, 0]))
-    assert reg.sum() > 400
-    assert worst <= REL_TOL
     # dp and z columns are carried unchanged
     alive = ~np.isnan(gpu[:, -1, 0])
     assert np.all(gpu[alive, :, 5] == bunch[alive, None, 4])
@@ -68,15 +81,16 @@ def test_dba_stable_aperture_subset():
 
 
 def test_dba_track_coordinates():
-    """DA ring tracked 256 turns from grid points: coordinates within 1e-12
-    of the oracle on regular particles."""
+    """DA ring tracked 64 turns from grid points (the ring sits on a
+    third-integer resonance: rounding is amplified quickly, so fewer turns
+    keep a quarter of the grid regular)."""
     ring = lattices.dba_ring(16)
     flags = ufo.KICK | ufo.DOUBLE_PRECISION | ufo.FIVED
     pool = lattices.da_grid(16, 4)
-    gpu, strict, fast = _track_both(ring, pool, flags, 256, ["x", "y"], [-1])
-    worst, reg, flags_ok = compare_tracks(gpu, strict, fast)
-    print(f"regular {reg.sum()}/{len(reg)}, worst rel err {worst:.3e}")
-    assert flags_ok and reg.sum() >= len(reg) // 2 and worst <= REL_TOL
+    gpu, strict, fast, moved = _track_both(ring, pool, flags, 64, ["x", "y"], [-1])
+    r = compare_tracks(gpu, strict, fast, moved)
+    print(summary(r))
+    assert_parity(r)
 
 
 def test_alba_thick_track_fp64():
@@ -87,10 +101,10 @@ def test_alba_thick_track_fp64():
     pool = np.stack([rng.normal(0, 2e-3, 64), rng.normal(0, 1e-4, 64),
                      rng.normal(0, 1e-3, 64), rng.normal(0, 1e-4, 64)], 1)
     flags = ufo.DOUBLE_PRECISION | ufo.FIVED
-    gpu, strict, fast = _track_both(alba.RING, pool, flags, 50, ["x", "px", "y", "py"], [-1], dp=1e-3)
-    worst, reg, flags_ok = compare_tracks(gpu, strict, fast)
-    print(f"regular {reg.sum()}/{len(reg)}, worst rel err {worst:.3e}")
-    assert flags_ok and reg.sum() >= 32 and worst <= REL_TOL
+    gpu, strict, fast, moved = _track_both(alba.RING, pool, flags, 50, ["x", "px", "y", "py"], [-1], dp=1e-3)
+    r = compare_tracks(gpu, strict, fast, moved)
+    print(summary(r))
+    assert_parity(r)
 
 
 def _all_elements_line():
@@ -127,10 +141,10 @@ def test_every_element_kind(flags):
     pool = np.stack([rng.normal(0, 2e-3, n), rng.normal(0, 2e-4, n), rng.normal(0, 2e-3, n),
                      rng.normal(0, 2e-4, n), rng.normal(0, 1e-3, n)], 1)
     where = [0.5, 2, 2.25, 4.5, 11, -1]
-    gpu, strict, fast = _track_both(line, pool, flags, 40, COORD6, where, dp=2e-3)
-    worst, reg, flags_ok = compare_tracks(gpu, strict, fast)
-    print(f"flags {flags}: regular {reg.sum()}/{n}, worst rel err {worst:.3e}")
-    assert flags_ok and reg.sum() >= n // 2 and worst <= REL_TOL
+    gpu, strict, fast, moved = _track_both(line, pool, flags, 40, COORD6, where, dp=2e-3)
+    r = compare_tracks(gpu, strict, fast, moved)
+    print(f"flags {flags}: " + summary(r))
+    assert_parity(r)
 
 
 def test_lattice_parameters_bound_per_particle():
@@ -144,9 +158,10 @@ def test_lattice_parameters_bound_per_particle():
                      rng.normal(0, 0.5, n), 0.8 + rng.normal(0, 0.1, n), rng.normal(0, 1e-4, n),
                      2.0 + rng.normal(0, 0.01, n)], 1)
     flags = ufo.KICK | ufo.DOUBLE_PRECISION | ufo.FIVED
-    gpu, strict, fast = _track_both(ring, pool, flags, 100, prm, [-1])
-    worst, reg, flags_ok = compare_tracks(gpu, strict, fast)
-    assert flags_ok and reg.sum() >= n // 2 and worst <= REL_TOL
+    gpu, strict, fast, moved = _track_both(ring, pool, flags, 100, prm, [-1])
+    r = compare_tracks(gpu, strict, fast, moved)
+    print(summary(r))
+    assert_parity(r)
 
 
 def test_fp32_thick_madx_vector():
@@ -195,3 +210,34 @@ def test_sa_threads_and_refill():
     assert np.array_equal(sa.lost, full)
     ref = runner.stable_aperture(ring, pool, flags=flags, turns=300, parameters=["x", "y"])
     assert np.array_equal(full, ref)
+
+
+@pytest.mark.parametrize("double", [False, True])
+def test_alba_stable_aperture_howto_grid(double):
+    """HOWTO.md:332-370 12x12 ALBA grid, 1000 turns: the HIP kernel gives the
+    oracle's loss turns on every particle whose strict and fast-math oracle
+    runs agree; in FP32 (the documented run) every cell documented as 0 or
+    1000 is reproduced."""
+    import json
+    g = json.load(open(os.path.join(os.path.dirname(OPTICS_DIR), "reference_values.json")))["alba_sa_grid_fp32"]
+    alba = ufo.Lattice(os.path.join(OPTICS_DIR, "alba.mad"))
+    n = 12
+    x = np.linspace(-0.04, 0.04, n)
+    xx, yy = np.meshgrid(x, x)
+    pool = np.stack([xx.ravel(), yy.ravel()], 1)
+    flags = ufo.FIVED | (ufo.DOUBLE_PRECISION if double else 0)
+    sa = ufo.StableAperture(alba.RING, flags=flags, turns=1000, particles=n * n, parameters=["x", "y"])
+    sa.parameters[:] = pool
+    sa.run()
+    strict = runner.stable_aperture(alba.RING, pool, flags=flags, turns=1000, parameters=["x", "y"])
+    fast = runner.stable_aperture(alba.RING, pool, flags=flags, turns=1000, parameters=["x", "y"], mode="fast")
+    regular = strict == fast
+    print(f"regular {regular.sum()}/{n * n}; gpu != strict on {np.sum(sa.lost != strict)}")
+    if double:
+        assert np.array_equal(sa.lost[regular], strict[regular])
+    else:
+        # FP32: chaotic escapes near the boundary differ by a turn between
+        # any two FP32 evaluations; the documented 0/1000 cells must agree
+        ref = np.array(g["lost"]).ravel()
+        robust = (ref == 0) | (ref == 1000)
+        assert np.all(sa.lost[robust] == ref[robust])

@@ -112,9 +112,16 @@ class Env:
             return f"(-{self.val(e.t[1])})"
         name = self.invariants.get(e)
         if name is None:
+            # children first, so every shared sub-expression is computed once
+            if k == "f":
+                text = f"{e.t[1]}({self.val(e.t[2])})"
+            elif k == "?":
+                text = f"({self.val(e.t[1])} ? {self.val(e.t[2])} : {self.val(e.t[3])})"
+            else:
+                text = f"({self.val(e.t[1])} {k} {self.val(e.t[2])})"
             name = f"I{len(self.invariants)}"
             self.invariants[e] = name
-            self.inv_code.append(f"{name} = {self.raw(e)};")
+            self.inv_code.append(f"{name} = {text};")
         return name
 
     def neg(self, e) -> str:
@@ -282,14 +289,56 @@ def _chain(terms, add):
     return acc
 
 
+def split_exact(e: Expr, env: Env):
+    """(hi, lo) of a constant coefficient whose last operation is done
+    exactly: hi + lo equals the exact product/quotient of the two rounded
+    operands, so the per-turn product carries no systematic rounding bias
+    (the reference multiplies and divides at every turn,
+    ``S * px0 / k2``).  Non-constant or single-leaf coefficients: lo = None."""
+    f = env.f(e)
+    if not f.is_const() or env.fp32:
+        return f, None
+    t = Expr.of(e).t
+    sign = 1.0
+    while t[0] in ("neg", "?"):
+        if t[0] == "neg":
+            sign = -sign
+            t = t[1].t
+            continue
+        c = env.f(t[1])
+        if not c.is_const():
+            return f, None
+        t = (t[2] if c.value != 0.0 else t[3]).t
+    if t[0] in ("*", "/"):
+        a, b = env.f(t[1]), env.f(t[2])
+        if a.is_const() and b.is_const() and b.value != 0.0 and np.isfinite(a.value) and np.isfinite(b.value):
+            import mpmath
+            with mpmath.workprec(200):
+                x = mpmath.mpf(a.value) * mpmath.mpf(b.value) if t[0] == "*" else \
+                    mpmath.mpf(a.value) / mpmath.mpf(b.value)
+                hi = float(x)
+                lo = float(x - mpmath.mpf(hi))
+            if hi == sign * f.value * sign and lo != 0.0:
+                return ir.const(sign * hi), ir.const(sign * lo)
+    return f, None
+
+
 def _thick_plane(env: Env, u: str, pu: str, m11, m12, m21, m22, du=None, dpu=None):
-    """u' = m11 u + m12 pu (+ du); pu' = m21 u + m22 pu (+ dpu)."""
+    """u' = m11 u + m12 pu (+ du); pu' = m21 u + m22 pu (+ dpu); constant
+    coefficients carry their exact low parts."""
     a = env.new_tmp()
-    v11, v12, v21, v22 = (env.val(m) for m in (m11, m12, m21, m22))
-    first = f"{pu} * {v12}" if du is None else _fma(pu, v12, env.val(du))
-    second = f"{pu} * {v22}" if dpu is None else _fma(pu, v22, env.val(dpu))
-    return (f"{{ ufloat {a} = {u}; {u} = {_fma(a, v11, first)}; "
-            f"{pu} = {_fma(a, v21, second)}; }}")
+    (h11, l11), (h12, l12), (h21, l21), (h22, l22) = (split_exact(m, env) for m in (m11, m12, m21, m22))
+
+    def row(hu, lu, hp, lp, add):
+        acc = None if add is None else env.val(add)
+        for coef, var_ in ((lp, pu), (lu, a)):
+            if coef is not None:
+                acc = _fma(env.val(coef), var_, acc) if acc is not None else f"({var_} * {env.val(coef)})"
+        acc = _fma(pu, env.val(hp), acc) if acc is not None else f"{pu} * {env.val(hp)}"
+        return _fma(a, env.val(hu), acc)
+
+    return (f"{{ ufloat {a} = {u}; {u} = {row(h11, l11, h12, l12, du)}; "
+            f"{pu} = {row(h21, l21, h22, l22, dpu)}; }}")
 
 
 
@@ -342,7 +391,9 @@ def emit_thick_bend(op: ir.ThickBend, env: Env, out: List[str]):
         kx = k + h * h / ir.OODPPO
     else:
         kx = k + h * h
-    my = _bend_plane(env, k, op.L, None)
+    # vertical focusing strength is -k: k > 0 defocuses in y (cosh/sinh),
+    # k < 0 focuses (cos/sin), ufo/methods.py:157-168
+    my = _bend_plane(env, -k, op.L, None)
     mx = _bend_plane(env, kx, op.L, ir.DP * h)
     dx = None if is_zero(env.f(mx[4])) else mx[4]
     dpx = None if is_zero(env.f(mx[5])) else mx[5]

@@ -11,7 +11,7 @@
     periods x a 16.9 m double-bend period of 25 elements: 2 combined-function
     sector bends (L=1.384 m, angle 2 pi/32, k1=-0.4 m^-2, 8 slices), 6
     quadrupoles (L 0.25-0.5 m, |k1| 1.0-2.2 m^-2, 8 slices), 4 sextupoles
-    (L=0.2 m, k2 = +-40 m^-3, 4 slices) and 13 drifts; an aperture
+    (L=0.2 m, k2 = +-38 m^-3, 4 slices) and 13 drifts; an aperture
     |x|,|y| < 30 mm at the end of the ring.  Per period the linear tunes are
     0.604 (x) and 0.243 (y), so 16 periods sit at Qx = 9.664, on the
     third-integer resonance driven by the sextupoles: the dynamic-aperture
@@ -48,7 +48,7 @@ def fodo_ring(cells=16, aperture=0.04, distinct=False):
 
 DBA_QUADS = {"QF1": (0.25, 2.2), "QD1": (0.30, -1.0), "QF2": (0.50, 1.5),
              "QF3": (0.50, 1.5), "QD2": (0.30, -1.0), "QF4": (0.25, 2.2)}
-DBA_SEXTS = {"SD1": -40.0, "SF1": 40.0, "SF2": 40.0, "SD2": -40.0}
+DBA_SEXTS = {"SD1": -38.0, "SF1": 38.0, "SF2": 38.0, "SD2": -38.0}
 
 
 def dba_period(tag=""):
