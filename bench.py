@@ -11,7 +11,7 @@ and apertures excluded).
 
 Also reported:
   roofline   the tracking kernel's canonical FP64 work per launch (executed
-             particle-turns x 13952 FLOP, the UFO FLOP rule) over its average
+             particle-turns x 13888 FLOP, the UFO FLOP rule) over its average
              launch time from HIP events on the launch stream, against the
              MI355X FP64 peak of 78.6 TFLOP/s;
   cpu_baseline  the oracle (the reference's kernels restated in C, gcc
@@ -42,6 +42,9 @@ def main():
     ap.add_argument("--turns", type=int, default=1024)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--calibrate-copy", type=int, default=0,
+                    help="MiB copied by one device-to-device copy before the timed region "
+                         "(reference kernel for HBM traffic counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1",
@@ -67,6 +70,12 @@ def main():
     E = sa.kernel.elements
     flop_pt = sa.kernel.flops_per_turn
 
+    if args.calibrate_copy:
+        src = torch.ones(args.calibrate_copy << 17, dtype=torch.float64, device=sa.device)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     for _ in range(args.warmup):
         sa.launch()
     torch.cuda.synchronize()
@@ -123,7 +132,7 @@ def main():
             "peak": PEAK_FP64_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved_tflops / PEAK_FP64_TFLOPS,
-            "traffic": None,
+            "traffic": measured_traffic(args),
             "kernel": sa.kernel.name,
             "kernel_ms": float(np.mean(kernel_ms)),
         },
@@ -132,6 +141,22 @@ def main():
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(ring, flags, pool, args.turns, E, args.cpu_seconds)
     print(json.dumps(out))
+
+
+def measured_traffic(args):
+    """HBM bytes per launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
+    summary of this workload (copy-calibrated), when the run is that
+    workload; None otherwise."""
+    if (args.nx, args.ny, args.turns) != (1024, 1024, 1024):
+        return None
+    path = os.path.join(ROOT, "profiles", "ufo_sa_dba_da_scan_1M_1024turns.md")
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch"):
+                return int(line.split("=")[1])
+    except OSError:
+        return None
+    return None
 
 
 def cpu_baseline(ring, flags, pool, turns, E, budget_s):
