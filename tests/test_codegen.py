@@ -96,3 +96,27 @@ def test_aperture_window_spliced_verbatim():
     line = ufo.Line("l", [ufo.Aperture("ap", window="x > 0.1 && y > 0.1")])
     k = codegen.track_kernel(line, ufo.DOUBLE_PRECISION, 1, ["x", "y"], [-1], 0.0)
     assert "if (x > 0.1 && y > 0.1 == 0) x = UFO_NAN;" in k.src
+
+
+def test_strided_period_for_per_magnet_parameters():
+    """Per-sextupole parameters (the HOWTO closed-orbit example) keep the
+    4-cell loop of ALBA: repetitions differ by a parameter stride of 30."""
+    from ufo_amd import tangent
+    alba = ufo.Lattice(os.path.join(OPTICS_DIR, "alba.mad"))
+    sext = alba.RING.find(lambda e: isinstance(e, ufo.Sextupole))
+    ops = [op for piece in alba.RING.method(flags=ufo.FIVED, parameters=[(e, "dx") for e in sext])
+           for op in piece]
+    head, p, reps, stride = codegen.find_period(ops, with_stride=True)
+    assert reps == 4 and stride == 30
+    k = tangent.co_kernel(alba.RING, ufo.FIVED | ufo.DOUBLE_PRECISION, [(e, "dx") for e in sext], 64, 1e-15)
+    assert "row[" in k.src and "rep * 30]" in k.src
+    runtime.compile_only(k.src, codegen.HIPRTC_OPTIONS, "gfx950")
+
+
+def test_tangent_kernels_compile():
+    from ufo_amd import tangent
+    line = _all_elements_line()
+    for fl in (ufo.KICK | ufo.DOUBLE_PRECISION, ufo.DOUBLE_PRECISION | ufo.EXACT, ufo.DOUBLE_PRECISION | ufo.FIVED):
+        runtime.compile_only(tangent.jvp_kernel(line, fl, 2, ["x", "px", "y", "py", "dp"], 1e-3).src,
+                             codegen.HIPRTC_OPTIONS, "gfx950")
+        runtime.compile_only(tangent.co_kernel(line, fl, ["x"], 20, 1e-15).src, codegen.HIPRTC_OPTIONS, "gfx950")

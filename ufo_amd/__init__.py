@@ -35,9 +35,9 @@ def __getattr__(name):
     if name in ("StableAperture",):
         from .sa import StableAperture
         return StableAperture
-    if name in ("ClosedOrbit",):
-        from .co import ClosedOrbit
-        return ClosedOrbit
+    if name in ("ClosedOrbit", "TangentMap"):
+        from . import co
+        return getattr(co, name)
     if name in ("Optics",):
         from .optics import Optics
         return Optics

@@ -69,6 +69,8 @@ class Env:
     inv_code: List[str] = field(default_factory=list)
     params_used: set = field(default_factory=set)
     tmp: int = 0
+    param_mem: bool = False          # parameters read from the pool row at use
+    stride: Optional[tuple] = None   # (loop variable, parameter stride) inside a strided period loop
 
     def f(self, e) -> Expr:
         return fold(e, self.fp32, self.consts)
@@ -83,6 +85,10 @@ class Env:
         if k == "c":
             return self.lit(t[1])
         if k == "p":
+            if self.stride is not None and self.stride[1] != 0:
+                return f"row[{t[1]} + {self.stride[0]} * {self.stride[1]}]"
+            if self.param_mem:
+                return f"row[{t[1]}]"
             self.params_used.add(t[1])
             return f"P{t[1]}"
         if k == "v":
@@ -107,6 +113,8 @@ class Env:
         if k == "p":
             return self.raw(e)
         if e.has_raw() or e.uses_var("turn"):
+            return self.raw(e)
+        if e.params() and (self.param_mem or (self.stride is not None and self.stride[1] != 0)):
             return self.raw(e)
         if k == "neg":
             return f"(-{self.val(e.t[1])})"
@@ -343,9 +351,10 @@ def _thick_plane(env: Env, u: str, pu: str, m11, m12, m21, m22, du=None, dpu=Non
 
 
 
-def emit_thick_quad(op: ir.ThickQuad, env: Env, out: List[str]):
-    """ufo/methods.py:106-132: K = sqrt(|k1 oodppo|); focusing in x for
-    k1 > 0, else in y (k1 = 0 gives 0/0 = NaN, as in the reference)."""
+def thick_quad_planes(op: ir.ThickQuad):
+    """(x-plane, y-plane) matrices (m11, m12, m21, m22) of a thick quad,
+    ufo/methods.py:106-132: K = sqrt(|k1 oodppo|); focusing in x for k1 > 0,
+    else in y (k1 = 0 gives 0/0 = NaN, as in the reference)."""
     k = op.k1
     kk = k * ir.OODPPO if op.chroma else k
     k2 = ir.fn("sqrt", ir.fn("fabs", kk))
@@ -357,6 +366,11 @@ def emit_thick_quad(op: ir.ThickQuad, env: Env, out: List[str]):
     cond = ir.gt(k, ir.ZERO)
     fx = tuple(ir.select(cond, a, b) for a, b in zip(foc, dfc))
     fy = tuple(ir.select(cond, b, a) for a, b in zip(foc, dfc))
+    return fx, fy
+
+
+def emit_thick_quad(op: ir.ThickQuad, env: Env, out: List[str]):
+    fx, fy = thick_quad_planes(op)
     out.append(_thick_plane(env, "x", "px", *fx))
     out.append(_thick_plane(env, "y", "py", *fy))
 
@@ -380,9 +394,10 @@ def _bend_plane(env: Env, kexpr: Expr, L: Expr, disp: Optional[Expr]):
     return [ir.select(gt, p, ir.select(lt, n, z)) for p, n, z in zip(pos, neg, nul)]
 
 
-def emit_thick_bend(op: ir.ThickBend, env: Env, out: List[str]):
-    """ufo/methods.py:142-194: y plane with k = k1 (oodppo), x plane with
-    k + h^2 / oodppo and dispersion dp h (1-C)/|k|, dp h S/K."""
+def thick_bend_planes(op: ir.ThickBend):
+    """(x-plane matrix + dispersion (6 entries), y-plane matrix) of a thick
+    sector bend, ufo/methods.py:142-194: y plane with k = k1 (oodppo), x plane
+    with k + h^2 / oodppo and dispersion dp h (1-C)/|k|, dp h S/K."""
     h = op.angle / op.L
     k = op.k1
     if op.chroma:
@@ -393,8 +408,13 @@ def emit_thick_bend(op: ir.ThickBend, env: Env, out: List[str]):
         kx = k + h * h
     # vertical focusing strength is -k: k > 0 defocuses in y (cosh/sinh),
     # k < 0 focuses (cos/sin), ufo/methods.py:157-168
-    my = _bend_plane(env, -k, op.L, None)
-    mx = _bend_plane(env, kx, op.L, ir.DP * h)
+    my = _bend_plane(None, -k, op.L, None)
+    mx = _bend_plane(None, kx, op.L, ir.DP * h)
+    return mx, my
+
+
+def emit_thick_bend(op: ir.ThickBend, env: Env, out: List[str]):
+    mx, my = thick_bend_planes(op)
     dx = None if is_zero(env.f(mx[4])) else mx[4]
     dpx = None if is_zero(env.f(mx[5])) else mx[5]
     out.append(_thick_plane(env, "x", "px", mx[0], mx[1], mx[2], mx[3], dx, dpx))
@@ -405,16 +425,59 @@ def emit_thick_bend(op: ir.ThickBend, env: Env, out: List[str]):
 # Turn structure: period detection
 # ---------------------------------------------------------------------------
 
-def find_period(ops):
-    """Largest coverage (head, period, repeats) of the op list by a repeated
-    block starting within the first few ops; (0, 0, 0) when none repeats."""
+def _signature(op):
+    """An op with its parameter indices abstracted, and those indices in a
+    fixed order."""
+    found = []
+
+    def leaf(e):
+        if e.t[0] == "p":
+            found.append(e.t[1])
+            return Expr(("p", -1))
+        return e
+
+    sig = op.map_exprs(lambda e: ir.map_leaves(e, leaf))
+    return sig, tuple(found)
+
+
+def _param_stride(par, head, p, reps):
+    """Common per-repetition stride of the parameter indices of a candidate
+    period, or None when the repetitions do not differ by one stride."""
+    stride = None
+    for r in range(1, reps):
+        for j in range(p):
+            a, b = par[head + j], par[head + r * p + j]
+            if len(a) != len(b):
+                return None
+            for ia, ib in zip(a, b):
+                d = ib - ia
+                if d % r:
+                    return None
+                if stride is None:
+                    stride = d // r
+                elif d != stride * r:
+                    return None
+    return 0 if stride is None else stride
+
+
+def find_period(ops, with_stride=False):
+    """Largest coverage (head, period, repeats[, stride]) of the op list by a
+    repeated block starting within the first few ops.  Repetitions may
+    differ in their parameter indices by a constant stride per repetition
+    (per-magnet parameters of a ring of identical cells).  (0, 0, 0[, 0])
+    when nothing repeats."""
     n = len(ops)
+    none = (0, 0, 0, 0) if with_stride else (0, 0, 0)
     if n < 2:
-        return 0, 0, 0
+        return none
     ids = {}
-    seq = np.array([ids.setdefault(op, len(ids)) for op in ops], dtype=np.int64)
-    best = (0, 0, 0)
-    best_cov = 0
+    sigs, par = [], []
+    for op in ops:
+        sg, pr = _signature(op)
+        sigs.append(ids.setdefault(sg, len(ids)))
+        par.append(pr)
+    seq = np.array(sigs, dtype=np.int64)
+    cands = []
     for head in range(0, min(9, n - 1)):
         s = seq[head:]
         m = len(s)
@@ -423,12 +486,17 @@ def find_period(ops):
             bad = np.flatnonzero(~eq)
             run = (len(eq) if len(bad) == 0 else bad[0]) + p
             reps = run // p
-            if reps >= 2 and reps * p > best_cov:
-                best_cov = reps * p
-                best = (head, p, reps)
-        if best_cov == m:
-            break
-    return best
+            if reps >= 2:
+                cands.append((reps * p, -head, -p, head, p, reps))
+    cands.sort(reverse=True)
+    for cov, _, _, head, p, reps in cands:
+        # shrink the repeat count until the parameter indices follow a stride
+        while reps >= 2:
+            st = _param_stride(par, head, p, reps)
+            if st is not None:
+                return (head, p, reps, st) if with_stride else (head, p, reps)
+            reps -= 1
+    return none
 
 
 # ---------------------------------------------------------------------------
@@ -458,26 +526,32 @@ class Kernel:
     period: tuple = (0, 0, 0)
 
 
-def _layout(body_ops, env, observe=None, loop_var="rep"):
+def _layout(body_ops, env, observe=None, loop_var="rep", emit=None):
     """Emit the op list, as a loop over the repeated period when there is
-    one."""
-    head, p, reps = find_period(body_ops) if len(body_ops) > 8 else (0, 0, 0)
+    one.  ``emit(op, env, out, observe)`` writes one op (default: the
+    forward pass)."""
+    emit = emit or emit_op
+    head, p, reps, stride = find_period(body_ops, True) if len(body_ops) > 8 else (0, 0, 0, 0)
     out: List[str] = []
     if reps >= 2:
         for op in body_ops[:head]:
-            emit_op(op, env, out, observe)
+            emit(op, env, out, observe)
         out.append("#pragma clang loop unroll(disable)")
         out.append(f"for (int {loop_var} = 0; {loop_var} < {reps}; ++{loop_var}) {{")
         inner: List[str] = []
-        for op in body_ops[head:head + p]:
-            emit_op(op, env, inner, observe)
+        env.stride = (loop_var, stride)
+        try:
+            for op in body_ops[head:head + p]:
+                emit(op, env, inner, observe)
+        finally:
+            env.stride = None
         out += ["    " + s for s in inner]
         out.append("}")
         for op in body_ops[head + p * reps:]:
-            emit_op(op, env, out, observe)
+            emit(op, env, out, observe)
     else:
         for op in body_ops:
-            emit_op(op, env, out, observe)
+            emit(op, env, out, observe)
     return out, (head, p, reps)
 
 
@@ -493,7 +567,18 @@ def _coord_sources(parameters, flags, dp):
     return src
 
 
-def _env_for(flags, src):
+PARAM_REGISTERS = 24   # at most this many parameters are kept in registers per particle
+
+
+def count_params(ops):
+    used = set()
+    for op in ops:
+        for e in op.exprs():
+            used |= e.params()
+    return used
+
+
+def _env_for(flags, src, ops=None):
     fp32 = not (flags & DOUBLE_PRECISION)
     consts = {}
     if src["dp"][0] == "c":
@@ -504,7 +589,10 @@ def _env_for(flags, src):
             consts["oodppo"] = float(one / (np.float32(dpv) + one))
         else:
             consts["oodppo"] = 1.0 / (dpv + 1.0)
-    return Env(fp32=fp32, consts=consts)
+    env = Env(fp32=fp32, consts=consts)
+    if ops is not None and len(count_params(ops)) > PARAM_REGISTERS:
+        env.param_mem = True
+    return env
 
 
 def _load_block(env: Env, src, P: int, row: str, indent: str):
@@ -554,7 +642,7 @@ def sa_kernel(line, flags, turns, parameters, dp, block=256, name="ufo_sa") -> K
     pieces = line.method(flags=flags, parameters=parameters)
     ops = [op for piece in pieces for op in piece]
     src = _coord_sources(parameters, flags, dp)
-    env = _env_for(flags, src)
+    env = _env_for(flags, src, ops)
     body, period = _layout(ops, env)
     P = max(1, len(parameters))
     one = env.lit(1.0)
@@ -572,10 +660,11 @@ extern "C" __global__ void __launch_bounds__({block})
     int idx = blockIdx.x * {block} + threadIdx.x;
     int turn = 0;
     bool need = true;
+    const ufloat *__restrict__ row = pool;
     for (;;) {{
         if (need) {{
             if (idx >= particles) break;
-            const ufloat *__restrict__ row = pool + (size_t)idx * {P};
+            row = pool + (size_t)idx * {P};
 {chr(10).join(load)}
             turn = 0;
             need = false;
@@ -620,7 +709,7 @@ def track_kernel(line, flags, turns, parameters, where, dp, block=256, name="ufo
         ops.append(ir.Observe())
     dumps = sum(1 for op in ops if isinstance(op, ir.Observe))
     src = _coord_sources(parameters, flags, dp)
-    env = _env_for(flags, src)
+    env = _env_for(flags, src, ops)
     store = ("{ size_t b = (size_t)s * 6 * np + idx; "
              "tracks[b] = x; tracks[b + np] = px; tracks[b + 2 * np] = y; "
              "tracks[b + 3 * np] = py; tracks[b + 4 * np] = z; tracks[b + 5 * np] = dp; ++s; }")
@@ -702,4 +791,4 @@ extern "C" __global__ void __launch_bounds__(64)
 
 
 HIPRTC_OPTIONS = ("-O3 -std=c++17 -ffp-contract=off -Xclang -target-feature -Xclang -packed-fp32-ops "
-                  "-Wno-bool-compare -Wno-unused-variable -Wno-unused-but-set-variable")
+                  "-Wno-unused-variable -Wno-unused-but-set-variable")

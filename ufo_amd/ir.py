@@ -132,6 +132,14 @@ class Expr:
         return any(e.t == ("v", name) for e in self.leaves())
 
 
+def map_leaves(e: Expr, fn_) -> Expr:
+    """Copy of ``e`` with every leaf replaced by ``fn_(leaf)``."""
+    t = e.t
+    if t[0] in ("c", "p", "v", "r"):
+        return fn_(e)
+    return Expr(tuple(map_leaves(c, fn_) if isinstance(c, Expr) else c for c in t))
+
+
 def select(cond, a, b) -> Expr:
     """``cond ? a : b`` (cond is a comparison expression)."""
     return Expr(("?", Expr.of(cond), Expr.of(a), Expr.of(b)))
