@@ -41,6 +41,9 @@ def __getattr__(name):
     if name in ("Optics",):
         from .optics import Optics
         return Optics
+    if name in ("AdjointTracker",):
+        from .adjoint import AdjointTracker
+        return AdjointTracker
     if name in ("list_devices", "context", "DEFAULT_CONTEXT"):
         from . import runtime
         if name == "DEFAULT_CONTEXT":

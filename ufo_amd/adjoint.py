@@ -1,0 +1,502 @@
+"""Reverse-mode adjoint of the tracker: gradients of a final-state loss with
+respect to shared lattice parameters (quadrupole k1, sextupole and multipole
+strengths, misalignments, lengths) and to the initial coordinates.
+
+Forward kernel ``ufo_adj_fwd``: tracks every particle ``turns`` turns with
+the lattice parameters read from a shared vector theta (wave-uniform scalar
+loads), stores the state every ``every`` turns (checkpoints, structure of
+arrays [checkpoint][coordinate][particle]), the final state and the loss
+turn.
+
+Reverse kernel ``ufo_adj_bwd``: per surviving particle, from lambda =
+dL/d(final state), walks the checkpoint blocks backwards.  Inside a block the
+intermediate states are recomputed by the exact inverse of every op (drift:
+x -= px L; kick: px += Re w, py -= Im w; thick: the inverse matrix), starting
+from the block's end checkpoint; each block starts again from its stored
+checkpoint, so reconstruction errors never build up over more than
+``every`` turns.  Per op, in reverse order: restore the pre-op state,
+accumulate the parameter gradient (the op's coefficient adjoint times the
+symbolic derivative of the coefficient with respect to the parameter), then
+lambda <- J^T lambda.
+
+Gradient accumulation: contributions of one parameter within one period are
+summed in registers and flushed once per period iteration into per-wave LDS
+accumulators (16 slots per parameter, lane % 16, hardware FP64 LDS atomics);
+at the end each block writes its per-parameter partial sums.  Lost particles
+(the loss ignores them) do not enter the reverse pass.
+"""
+from __future__ import annotations
+
+import math
+from typing import Dict, List
+
+import numpy as np
+import torch
+
+from . import codegen, ir
+from .codegen import Env, _fma, _negs, is_zero, emit_op, kick_coeffs
+from .flags import DOUBLE_PRECISION
+from .sim import Simulation
+
+ACCW = 16   # LDS accumulator slots per parameter (lane % ACCW)
+
+
+class _Grad:
+    """Gradient registers of one emitted segment: param column -> name."""
+
+    def __init__(self):
+        self.regs: Dict[int, str] = {}
+
+    def reg(self, i: int) -> str:
+        if i not in self.regs:
+            self.regs[i] = f"G{i}"
+        return self.regs[i]
+
+
+def _add_grad(env: Env, g: _Grad, coef: ir.Expr, adj: str, out: List[str]):
+    """G_i += adj * d coef / d theta_i for every shared parameter i in coef."""
+    for i in sorted(ir.Expr.of(coef).params()):
+        if i < env.shared_offset:
+            continue
+        d = env.f(ir.diff(coef, i))
+        if is_zero(d):
+            continue
+        r = g.reg(i)
+        out.append(f"{r} = fma({adj}, {env.val(d)}, {r});")
+
+
+def emit_reverse(op, env: Env, out: List[str], g: _Grad):
+    f = env.f
+    if isinstance(op, ir.Drift):
+        L = f(op.L)
+        if is_zero(L):
+            return
+        v = env.val(L)
+        out.append(f"x = fma(-px, {v}, x); y = fma(-py, {v}, y);")
+        _add_grad(env, g, op.L, "fma(lx, px, ly * py)", out)
+        out.append(f"lpx = fma(lx, {v}, lpx); lpy = fma(ly, {v}, lpy);")
+    elif isinstance(op, ir.ExactDrift):
+        L = env.val(op.L)
+        opdp2 = env.val(ir.E(1.0 + ir.DP) * (1.0 + ir.DP))
+        one = env.lit(1.0)
+        out.append(f"{{ ufloat ia = {one} / sqrt({_fma('-py', 'py', _fma('-px', 'px', opdp2))}); "
+                   f"ufloat l = {L} * ia; ufloat c = l * ia * ia; "
+                   f"x = fma(-px, l, x); y = fma(-py, l, y); "
+                   f"ufloat s = c * fma(lx, px, ly * py); "
+                   f"lpx = fma(lx, l, fma(px, s, lpx)); lpy = fma(ly, l, fma(py, s, lpy)); }}")
+        if ir.Expr.of(op.L).params():
+            raise NotImplementedError("gradient with respect to an exact-drift length")
+    elif isinstance(op, ir.Kick):
+        emit_kick_reverse(op, env, out, g)
+    elif isinstance(op, ir.Weak):
+        w = f(op.w)
+        if not is_zero(w):
+            v = env.val(w)
+            out.append(f"px = fma({v}, x, px);")
+            _add_grad(env, g, op.w, "(-lpx * x)", out)
+            out.append(f"lx = fma({env.neg(w)}, lpx, lx);")
+    elif isinstance(op, ir.Shift):
+        dx, dy = f(op.dx), f(op.dy)
+        if not is_zero(dx):
+            out.append(f"x = x + {env.val(dx)};")
+        if not is_zero(dy):
+            out.append(f"y = y + {env.val(dy)};")
+        _add_grad(env, g, op.dx, "(-lx)", out)
+        _add_grad(env, g, op.dy, "(-ly)", out)
+    elif isinstance(op, ir.Edge):
+        h = op.h0 * ir.OODPPO if op.chroma else op.h0
+        t1e = h * ir.fn("tan", op.e)
+        psi = op.e - h * 2.0 * op.fringe / ir.fn("cos", op.e) * (ir.E(1.0) + ir.fn("sin", op.e) * ir.fn("sin", op.e))
+        t2e = h * ir.fn("tan", psi)
+        t1, t2 = f(t1e), f(t2e)
+        if not is_zero(t1):
+            out.append(f"px = fma(-x, {env.val(t1)}, px);")
+        if not is_zero(t2):
+            out.append(f"py = fma(y, {env.val(t2)}, py);")
+        _add_grad(env, g, t1e, "(lpx * x)", out)
+        _add_grad(env, g, t2e, "(-lpy * y)", out)
+        if not is_zero(t1):
+            out.append(f"lx = fma({env.val(t1)}, lpx, lx);")
+        if not is_zero(t2):
+            out.append(f"ly = fma({env.neg(t2)}, lpy, ly);")
+    elif isinstance(op, (ir.ThickQuad, ir.ThickBend)):
+        emit_thick_reverse(op, env, out, g)
+    elif isinstance(op, ir.Wire):
+        wx, wy, kk = env.val(op.wx), env.val(op.wy), env.val(op.k)
+        if any(ir.Expr.of(e).params() for e in (op.wx, op.wy, op.k)):
+            raise NotImplementedError("gradient with respect to wire parameters")
+        out.append(f"{{ ufloat a = {wx} - x, b = {wy} - y; ufloat r2 = fma(a, a, b * b); "
+                   f"ufloat kr = {kk} / r2; px = fma(-kr, a, px); py = fma(-kr, b, py); "
+                   f"ufloat q = kr / r2; ufloat faa = q * fma(b, b, -a * a); "
+                   f"ufloat fab = {env.lit(-2.0)} * q * a * b; "
+                   f"ufloat nlx = fma(-faa, lpx, fma(-fab, lpy, lx)); "
+                   f"ly = fma(-fab, lpx, fma(faa, lpy, ly)); lx = nlx; }}")
+    elif isinstance(op, ir.Aperture):
+        return
+    elif isinstance(op, (ir.Code, ir.Observe)):
+        raise NotImplementedError(f"{type(op).__name__} has no adjoint")
+    else:
+        raise TypeError(f"no adjoint for {op!r}")
+
+
+def emit_kick_reverse(op: ir.Kick, env: Env, out: List[str], g: _Grad):
+    """Inverse kick px += Re w, py -= Im w; lambda_x -= a lpx - b lpy,
+    lambda_y += b lpx + a lpy with w' = a + i b; coefficient adjoints
+    dL/d kn_n = (-lpx Re z^n + lpy Im z^n) chi / n!,
+    dL/d ks_n = (lpx Im z^n + lpy Re z^n) chi / n!  (chi = oodppo or 1)."""
+    d = kick_coeffs(op, env)
+    n = len(d)
+    top = -1
+    for i, (cr, ci) in enumerate(d):
+        if not is_zero(cr) or not is_zero(ci):
+            top = i
+    lines: List[str] = []
+    # w and w' at z (x, y unchanged by the kick)
+    if top >= 0:
+        wr = None if is_zero(d[top][0]) else env.val(d[top][0])
+        wi = None if is_zero(d[top][1]) else env.val(d[top][1])
+        if top >= 1:
+            dr = None if is_zero(env.f(d[top][0] * float(top))) else env.val(env.f(d[top][0] * float(top)))
+            di = None if is_zero(env.f(d[top][1] * float(top))) else env.val(env.f(d[top][1] * float(top)))
+        for lvl in range(top - 1, -1, -1):
+            cr = None if is_zero(d[lvl][0]) else env.val(d[lvl][0])
+            ci = None if is_zero(d[lvl][1]) else env.val(d[lvl][1])
+            if lvl >= 1:
+                er = env.f(d[lvl][0] * float(lvl))
+                ei = env.f(d[lvl][1] * float(lvl))
+                er = None if is_zero(er) else env.val(er)
+                ei = None if is_zero(ei) else env.val(ei)
+                nr = codegen._chain([(dr, "x", False), (di, "y", True)], er)
+                ni = codegen._chain([(dr, "y", False), (di, "x", False)], ei)
+                a, b = env.new_tmp(), env.new_tmp()
+                lines.append(f"ufloat {a} = {nr or env.lit(0.0)}; ufloat {b} = {ni or env.lit(0.0)};")
+                dr, di = a, b
+            nr = codegen._chain([(wr, "x", False), (wi, "y", True)], cr)
+            ni = codegen._chain([(wr, "y", False), (wi, "x", False)], ci)
+            a, b = env.new_tmp(), env.new_tmp()
+            lines.append(f"ufloat {a} = {nr or env.lit(0.0)}; ufloat {b} = {ni or env.lit(0.0)};")
+            wr, wi = a, b
+        if wr is not None:
+            lines.append(f"px = px + {wr};")
+        if wi is not None:
+            lines.append(f"py = py - {wi};")
+    # coefficient adjoints: powers of z
+    params_here = any(ir.Expr.of(c).params() for c in list(op.kn) + list(op.ks))
+    if params_here:
+        zr, zi = env.lit(1.0), env.lit(0.0)
+        for m in range(n):
+            if m > 0:
+                a, b = env.new_tmp(), env.new_tmp()
+                lines.append(f"ufloat {a} = fma({zr}, x, -({zi}) * y); ufloat {b} = fma({zr}, y, ({zi}) * x);")
+                zr, zi = a, b
+            chi = ir.OODPPO if op.chroma else ir.E(1.0)
+            scale = chi / float(math.factorial(m))
+            if m < len(op.kn) and ir.Expr.of(op.kn[m]).params():
+                _add_grad(env, g, op.kn[m] * scale, f"fma(-lpx, {zr}, lpy * {zi})", lines)
+            if m < len(op.ks) and ir.Expr.of(op.ks[m]).params():
+                _add_grad(env, g, op.ks[m] * scale, f"fma(lpx, {zi}, lpy * {zr})", lines)
+    if top >= 1:
+        # lambda update with w' = dr + i di
+        lx = "lx"
+        ly = "ly"
+        if dr is not None:
+            lx = _fma(_negs(dr), "lpx", lx)
+            ly = _fma(dr, "lpy", ly)
+        if di is not None:
+            lx = _fma(di, "lpy", lx)
+            ly = _fma(di, "lpx", ly)
+        lines.append(f"lx = {lx}; ly = {ly};")
+    if lines:
+        out.append("{ " + " ".join(lines) + " }")
+
+
+def emit_thick_reverse(op, env: Env, out: List[str], g: _Grad):
+    """x_post = M x_pre + d per plane (det M = 1): x_pre = M^-1 (x_post - d),
+    lambda_pre = M^T lambda_post."""
+    if isinstance(op, ir.ThickQuad):
+        mx, my = codegen.thick_quad_planes(op)
+        dx = dpx = None
+    else:
+        mx, my = codegen.thick_bend_planes(op)
+        dx = None if is_zero(env.f(mx[4])) else mx[4]
+        dpx = None if is_zero(env.f(mx[5])) else mx[5]
+    for e in list(mx) + list(my):
+        if any(i >= env.shared_offset for i in ir.Expr.of(e).params()):
+            raise NotImplementedError("gradient with respect to thick-element parameters")
+    for (u, pu, lu, lpu), m, du, dpu in ((("x", "px", "lx", "lpx"), mx, dx, dpx),
+                                         (("y", "py", "ly", "lpy"), my, None, None)):
+        v11, v12, v21, v22 = (env.val(e) for e in m[:4])
+        uu = f"({u} - {env.val(du)})" if du is not None else u
+        pp = f"({pu} - {env.val(dpu)})" if dpu is not None else pu
+        out.append(f"{{ ufloat a = {uu}, b = {pp}; {u} = fma({v22}, a, -({v12}) * b); "
+                   f"{pu} = fma({v11}, b, -({v21}) * a); "
+                   f"ufloat la = {lu}; {lu} = fma({v11}, la, {v21} * {lpu}); "
+                   f"{lpu} = fma({v12}, la, {v22} * {lpu}); }}")
+
+
+# ---------------------------------------------------------------------------
+# Kernels
+# ---------------------------------------------------------------------------
+
+def _turn_ops(line, flags, parameters):
+    pieces = line.method(flags=flags, parameters=parameters)
+    return [op for piece in pieces for op in piece]
+
+
+def _reverse_layout(ops, env: Env, ntheta: int):
+    """Reverse pass of one turn: the ops backwards, the repeated period as a
+    descending loop; gradient registers flushed to LDS per segment."""
+    head, p, reps, stride = codegen.find_period(ops, True) if len(ops) > 8 else (0, 0, 0, 0)
+
+    def segment(seg_ops, lv=None):
+        g = _Grad()
+        body: List[str] = []
+        for op in reversed(seg_ops):
+            emit_reverse(op, env, body, g)
+        decl = [f"ufloat {r} = {env.lit(0.0)};" for r in g.regs.values()]
+        flush = []
+        for i, r in sorted(g.regs.items()):
+            j = i - env.shared_offset
+            idx = f"({j} + {lv} * {stride})" if (lv and stride) else f"{j}"
+            flush.append(f"unsafeAtomicAdd(&acc[{idx} * {ACCW} + slot], {r});")
+        return ["{"] + ["    " + s for s in decl + body + flush] + ["}"]
+
+    out: List[str] = []
+    if reps >= 2:
+        out += segment(ops[head + p * reps:])
+        out.append("#pragma clang loop unroll(disable)")
+        out.append(f"for (int rep = {reps - 1}; rep >= 0; --rep) {{")
+        env.stride = ("rep", stride)
+        try:
+            out += ["    " + s for s in segment(ops[head:head + p], "rep")]
+        finally:
+            env.stride = None
+        out.append("}")
+        out += segment(ops[:head])
+    else:
+        out += segment(ops)
+    return out, (head, p, reps, stride)
+
+
+def adjoint_kernels(line, flags, turns, coords, lattice_parameters, dp=0.0, every=32, block=64):
+    """(forward kernel, reverse kernel, number of checkpoints) of a loss on
+    the final state.  ``coords`` are the per-particle coordinate columns
+    (names among x, px, y, py, dp), ``lattice_parameters`` the shared
+    (label-or-index, name) parameters, in the order of theta."""
+    if not flags & DOUBLE_PRECISION:
+        raise ValueError("the adjoint tracker computes in FP64 (DOUBLE_PRECISION)")
+    coords = list(coords)
+    shared = list(lattice_parameters)
+    parameters = coords + shared
+    ops = _turn_ops(line, flags, parameters)
+    for op in ops:
+        if isinstance(op, ir.Code) or any(e.uses_var("turn") or e.has_raw() for e in op.exprs()):
+            raise NotImplementedError("raw user code has no adjoint")
+    nck = (turns + every - 1) // every + 1
+    src = codegen._coord_sources(coords, flags, dp)
+
+    def make_env():
+        env = codegen._env_for(flags, src, None)
+        env.shared_offset = len(coords)
+        return env
+
+    P = max(1, len(coords))
+    nth = max(1, len(shared))
+    pre = codegen._PRELUDE.format(ufloat="double", nan='__builtin_nan("")')
+
+    # forward
+    env = make_env()
+    fbody, period = codegen._layout(ops, env)
+    load = codegen._load_block(env, src, P, "row", "    ")
+    store_ck = ("{ ufloat *c = ckpt + (size_t)ck * 4 * np + idx; c[0] = x; c[np] = px; c[2 * np] = y; "
+                "c[3 * np] = py; }")
+    fwd = pre + f"""
+extern "C" __global__ void __launch_bounds__(256)
+ufo_adj_fwd(const ufloat *__restrict__ pool, const ufloat *__restrict__ th, ufloat *__restrict__ ckpt,
+            ufloat *__restrict__ xout, int *__restrict__ lost, int particles)
+{{
+    {codegen._decls(env)}
+    const int idx = blockIdx.x * 256 + threadIdx.x;
+    if (idx >= particles) return;
+    const size_t np = (size_t)particles;
+    const ufloat *__restrict__ row = pool + (size_t)idx * {P};
+{chr(10).join(load)}
+    int ck = 0;
+    {store_ck}
+    int turn = 0;
+    for (; turn < {turns}; ++turn) {{
+{chr(10).join('        ' + s_ for s_ in fbody)}
+        if (isnan(x)) break;
+        if ((turn + 1) % {every} == 0 || turn + 1 == {turns}) {{ ++ck; {store_ck} }}
+    }}
+    ufloat *o = xout + (size_t)idx * 4;
+    o[0] = x; o[1] = px; o[2] = y; o[3] = py;
+    lost[idx] = turn;
+}}
+"""
+    fk = codegen.Kernel(name="ufo_adj_fwd", src=fwd, kind="adj_fwd", fp32=False,
+                        flops_per_turn=codegen._count(ops, env), elements=codegen.element_count(line), block=256,
+                        period=period)
+
+    # reverse
+    env = make_env()
+    rbody, rperiod = _reverse_layout(ops, env, nth)
+    load = codegen._load_block(env, src, P, "row", "        ")
+    bwd = pre + f"""
+extern "C" __global__ void __launch_bounds__({block})
+ufo_adj_bwd(const ufloat *__restrict__ pool, const ufloat *__restrict__ th, const ufloat *__restrict__ ckpt,
+            const ufloat *__restrict__ lam, const int *__restrict__ lost, ufloat *__restrict__ gx0,
+            ufloat *__restrict__ gpart, int particles)
+{{
+    __shared__ ufloat acc[{nth} * {ACCW}];
+    for (int i = threadIdx.x; i < {nth} * {ACCW}; i += {block}) acc[i] = 0.0;
+    __syncthreads();
+    const int idx = blockIdx.x * {block} + threadIdx.x;
+    const int slot = threadIdx.x % {ACCW};
+    if (idx < particles && lost[idx] == {turns}) {{
+        {codegen._decls(env)}
+        const size_t np = (size_t)particles;
+        const ufloat *__restrict__ row = pool + (size_t)idx * {P};
+{chr(10).join(load)}
+        ufloat lx = lam[(size_t)idx * 4], lpx = lam[(size_t)idx * 4 + 1];
+        ufloat ly = lam[(size_t)idx * 4 + 2], lpy = lam[(size_t)idx * 4 + 3];
+        for (int ck = {nck - 1}; ck >= 1; --ck) {{
+            const int t_end = (ck * {every} < {turns}) ? ck * {every} : {turns};
+            const int t_begin = (ck - 1) * {every};
+            const ufloat *c = ckpt + (size_t)ck * 4 * np + idx;
+            x = c[0]; px = c[np]; y = c[2 * np]; py = c[3 * np];
+            for (int turn = t_end - 1; turn >= t_begin; --turn) {{
+{chr(10).join('                ' + s_ for s_ in rbody)}
+            }}
+        }}
+        ufloat *o = gx0 + (size_t)idx * 4;
+        o[0] = lx; o[1] = lpx; o[2] = ly; o[3] = lpy;
+    }}
+    __syncthreads();
+    for (int j = threadIdx.x; j < {nth}; j += {block}) {{
+        ufloat s = 0.0;
+        for (int k = 0; k < {ACCW}; ++k) s += acc[j * {ACCW} + k];
+        gpart[(size_t)blockIdx.x * {nth} + j] = s;
+    }}
+}}
+"""
+    bk = codegen.Kernel(name="ufo_adj_bwd", src=bwd, kind="adj_bwd", fp32=False,
+                        flops_per_turn=codegen._count(ops, env), elements=codegen.element_count(line), block=block,
+                        period=rperiod)
+    return fk, bk, nck
+
+
+# ---------------------------------------------------------------------------
+# Driver and autograd
+# ---------------------------------------------------------------------------
+
+
+class AdjointTracker(Simulation):
+    """Tracks particles through ``line`` for ``turns`` turns with the lattice
+    parameters ``lattice_parameters`` (tuples (label-or-index, name), the
+    reference's parameter convention, ufo/lattice.py:109-110) taken from a
+    shared vector theta, and back-propagates a final-state loss.
+
+    ``coords``: per-particle initial-coordinate columns (names among x, px,
+    y, py, dp).  ``every``: turns between checkpoints.
+
+    ``forward(x0, theta)`` -> (final state [N, 4] (NaN rows for lost
+    particles), lost turn [N]);  ``backward(lam)`` with lam = dL/d(final
+    state) -> (dL/d(x, px, y, py)_0 [N, 4], dL/dtheta [P]).
+    """
+
+    def __init__(self, line, flags, turns, coords, lattice_parameters, dp=0.0, every=32, context=None,
+                 options=None):
+        self._init_common(flags, context, options)
+        self.turns = int(turns)
+        self.coords = list(coords)
+        self.lattice_parameters = list(lattice_parameters)
+        self.every = int(every)
+        self.kfwd, self.kbwd, self.ncheck = adjoint_kernels(line, flags, self.turns, self.coords,
+                                                            self.lattice_parameters, dp, self.every)
+        self.kernel = self.kfwd
+        self._mods = None
+        self._saved = None
+
+    def modules(self):
+        if self._mods is None:
+            self._mods = (self.ctx.module(self.kfwd.src, self.options), self.ctx.module(self.kbwd.src, self.options))
+        return self._mods
+
+    def _theta(self, theta):
+        th = torch.as_tensor(theta, dtype=torch.float64, device=self.device).contiguous()
+        if th.numel() == 0:
+            th = torch.zeros(1, dtype=torch.float64, device=self.device)
+        return th
+
+    def forward(self, x0, theta):
+        fwd, _ = self.modules()
+        x0 = torch.as_tensor(x0, dtype=torch.float64, device=self.device).contiguous()
+        n = x0.shape[0]
+        pool = x0 if x0.shape[1] > 0 else torch.zeros((n, 1), dtype=torch.float64, device=self.device)
+        th = self._theta(theta)
+        ckpt = torch.empty((self.ncheck, 4, n), dtype=torch.float64, device=self.device)
+        xf = torch.empty((n, 4), dtype=torch.float64, device=self.device)
+        lost = torch.empty((n,), dtype=torch.int32, device=self.device)
+        if n:
+            fwd.launch(self.kfwd.name, (n + 255) // 256, 256, [pool, th, ckpt, xf, lost, n])
+        self._saved = (pool, th, ckpt, lost)
+        return xf, lost
+
+    def backward(self, lam):
+        _, bwd = self.modules()
+        pool, th, ckpt, lost = self._saved
+        n = pool.shape[0]
+        lam = torch.nan_to_num(torch.as_tensor(lam, dtype=torch.float64, device=self.device), nan=0.0).contiguous()
+        block = self.kbwd.block
+        nb = max(1, (n + block - 1) // block)
+        npar = max(1, len(self.lattice_parameters))
+        gx0 = torch.zeros((n, 4), dtype=torch.float64, device=self.device)
+        gpart = torch.empty((nb, npar), dtype=torch.float64, device=self.device)
+        if n:
+            bwd.launch(self.kbwd.name, nb, block, [pool, th, ckpt, lam, lost, gx0, gpart, n])
+        else:
+            gpart.zero_()
+        return gx0, gpart.sum(dim=0)[:len(self.lattice_parameters)]
+
+
+class _TrackFunction(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, theta, x0, tracker):
+        xf, lost = tracker.forward(x0, theta)
+        ctx.tracker = tracker
+        ctx.ncoord = x0.shape[1]
+        ctx.coords = tracker.coords
+        ctx.mark_non_differentiable(lost)
+        return xf, lost
+
+    @staticmethod
+    def backward(ctx, grad_xf, grad_lost):
+        gx0, gth = ctx.tracker.backward(grad_xf)
+        g0 = torch.zeros((gx0.shape[0], ctx.ncoord), dtype=gx0.dtype, device=gx0.device)
+        for j, name in enumerate(ctx.coords):
+            if name in ("x", "px", "y", "py"):
+                g0[:, j] = gx0[:, ("x", "px", "y", "py").index(name)]
+        return gth, g0, None
+
+
+def track(tracker: AdjointTracker, theta, x0):
+    """Differentiable tracking: (final state [N, 4], lost turn [N]).
+    Gradients flow to ``theta`` (shared lattice parameters) and to the x,
+    px, y, py columns of ``x0`` (the dp column gets none)."""
+    return _TrackFunction.apply(theta, x0, tracker)
+
+
+def amplitude_loss(xf, aperture=0.04, weight=1e-5, sharpness=20.0, onset=0.25):
+    """Final-state objective: mean over particles of alive * (x^2 + y^2)
+    plus ``weight`` times a soft-survival penalty, per particle
+    alive * softplus(s (r^2/a^2 - onset)) / s + (1 - alive), which grows as a
+    surviving particle nears the aperture a and counts each lost particle as
+    1 (lost particles carry no gradient)."""
+    alive = torch.isfinite(xf[:, 0]) & torch.isfinite(xf[:, 2])
+    x = torch.where(alive, xf[:, 0], torch.zeros_like(xf[:, 0]))
+    y = torch.where(alive, xf[:, 2], torch.zeros_like(xf[:, 2]))
+    r2 = x * x + y * y
+    soft = torch.nn.functional.softplus(sharpness * (r2 / (aperture * aperture) - onset)) / sharpness
+    pen = torch.where(alive, soft, torch.ones_like(soft))
+    return torch.mean(r2) + weight * torch.mean(pen)

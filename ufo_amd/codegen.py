@@ -70,6 +70,7 @@ class Env:
     params_used: set = field(default_factory=set)
     tmp: int = 0
     param_mem: bool = False          # parameters read from the pool row at use
+    shared_offset: int = 1 << 30     # parameter columns >= this come from the shared vector th[]
     stride: Optional[tuple] = None   # (loop variable, parameter stride) inside a strided period loop
 
     def f(self, e) -> Expr:
@@ -84,6 +85,11 @@ class Env:
         k = t[0]
         if k == "c":
             return self.lit(t[1])
+        if k == "p" and t[1] >= self.shared_offset:
+            j = t[1] - self.shared_offset
+            if self.stride is not None and self.stride[1] != 0:
+                return f"th[{j} + {self.stride[0]} * {self.stride[1]}]"
+            return f"th[{j}]"
         if k == "p":
             if self.stride is not None and self.stride[1] != 0:
                 return f"row[{t[1]} + {self.stride[0]} * {self.stride[1]}]"
@@ -489,14 +495,18 @@ def find_period(ops, with_stride=False):
             if reps >= 2:
                 cands.append((reps * p, -head, -p, head, p, reps))
     cands.sort(reverse=True)
+    best, best_cov = none, 0
     for cov, _, _, head, p, reps in cands:
+        if cov <= best_cov:
+            break
         # shrink the repeat count until the parameter indices follow a stride
-        while reps >= 2:
+        while reps >= 2 and reps * p > best_cov:
             st = _param_stride(par, head, p, reps)
             if st is not None:
-                return (head, p, reps, st) if with_stride else (head, p, reps)
+                best, best_cov = ((head, p, reps, st) if with_stride else (head, p, reps)), reps * p
+                break
             reps -= 1
-    return none
+    return best
 
 
 # ---------------------------------------------------------------------------

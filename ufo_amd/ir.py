@@ -140,6 +140,50 @@ def map_leaves(e: Expr, fn_) -> Expr:
     return Expr(tuple(map_leaves(c, fn_) if isinstance(c, Expr) else c for c in t))
 
 
+def diff(e: Expr, i: int) -> Expr:
+    """Symbolic derivative of ``e`` with respect to parameter column ``i``
+    (raw C text and kernel variables are constants)."""
+    e = Expr.of(e)
+    t = e.t
+    k = t[0]
+    if k in ("c", "v", "r"):
+        return ZERO
+    if k == "p":
+        return const(1.0) if t[1] == i else ZERO
+    if k == "neg":
+        return -diff(t[1], i)
+    if k == "+":
+        return diff(t[1], i) + diff(t[2], i)
+    if k == "-":
+        return diff(t[1], i) - diff(t[2], i)
+    if k == "*":
+        return diff(t[1], i) * t[2] + t[1] * diff(t[2], i)
+    if k == "/":
+        return diff(t[1], i) / t[2] - t[1] * diff(t[2], i) / (t[2] * t[2])
+    if k in (">", "<"):
+        return ZERO
+    if k == "?":
+        return select(t[1], diff(t[2], i), diff(t[3], i))
+    if k == "f":
+        a, da = t[2], diff(t[2], i)
+        name = t[1]
+        if name == "sqrt":
+            return da / (fn("sqrt", a) * 2.0)
+        if name == "fabs":
+            return select(gt(a, ZERO), da, -da)
+        if name == "sin":
+            return fn("cos", a) * da
+        if name == "cos":
+            return -(fn("sin", a) * da)
+        if name == "tan":
+            return da / (fn("cos", a) * fn("cos", a))
+        if name == "sinh":
+            return fn("cosh", a) * da
+        if name == "cosh":
+            return fn("sinh", a) * da
+    raise ValueError(f"cannot differentiate {e!r}")
+
+
 def select(cond, a, b) -> Expr:
     """``cond ? a : b`` (cond is a comparison expression)."""
     return Expr(("?", Expr.of(cond), Expr.of(a), Expr.of(b)))
