@@ -42,6 +42,7 @@ def main():
     ap.add_argument("--turns", type=int, default=1024)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="MiB copied by one device-to-device copy before the timed region "
                          "(reference kernel for HBM traffic counters)")
@@ -138,9 +139,55 @@ def main():
         },
         "cpu_baseline": None,
     }
+    if not args.no_adjoint:
+        out["adjoint"] = adjoint_timing()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(ring, flags, pool, args.turns, E, args.cpu_seconds)
     print(json.dumps(out))
+
+
+def adjoint_timing(cells=48, particles=65536, turns=512, every=32, reps=3):
+    """Gradient of the amplitude loss with respect to all 96 quadrupole k1 and
+    96 sextupole k2 of a 48-cell FODO ring (65 536 particles, sigma_x =
+    sigma_y = 3 mm, 512 turns, checkpoints every 32 turns): time of the
+    checkpointing forward pass and of the reverse pass (HIP events on the
+    launch stream), and their ratio."""
+    import numpy as np
+    import torch
+    import ufo_amd as ufo
+    from ufo_amd import adjoint, lattices
+    ring = lattices.fodo_ring(cells, distinct=True)
+    shared = ([(f"QF{i}", "k1") for i in range(cells)] + [(f"QD{i}", "k1") for i in range(cells)]
+              + [(f"SF{i}", "k2") for i in range(cells)] + [(f"SD{i}", "k2") for i in range(cells)])
+    theta = torch.tensor([1.2] * cells + [-1.2] * cells + [0.8] * cells + [-1.4] * cells, dtype=torch.float64)
+    bunch = lattices.gaussian_bunch(particles, seed=0, sx=3e-3, sy=3e-3)
+    tr = ufo.AdjointTracker(ring, ufo.KICK | ufo.DOUBLE_PRECISION, turns, ["x", "px", "y", "py", "dp"], shared,
+                            every=every)
+    th = theta.to(tr.device).requires_grad_(True)
+    x0 = torch.tensor(bunch, device=tr.device)
+    stream = torch.cuda.current_stream()
+    fw, bw = [], []
+    for k in range(reps + 1):
+        e = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+        e[0].record(stream)
+        xf, lost = tr.forward(x0, th.detach())
+        e[1].record(stream)
+        lam = torch.zeros_like(xf)
+        alive = torch.isfinite(xf[:, 0])
+        lam[alive] = 2.0 * xf[alive] * torch.tensor([1.0, 0.0, 1.0, 0.0], device=tr.device) / particles
+        gx0, gth = tr.backward(lam)
+        e[2].record(stream)
+        torch.cuda.synchronize()
+        if k:
+            fw.append(e[0].elapsed_time(e[1]))
+            bw.append(e[1].elapsed_time(e[2]))
+    f_ms, b_ms = float(np.mean(fw)), float(np.mean(bw))
+    pt = int(torch.clamp(lost.to(torch.int64) + 1, max=turns).sum())
+    flop = pt * tr.kfwd.flops_per_turn
+    return {"workload": f"{cells}-cell FODO, {particles} particles, {turns} turns, checkpoints every {every}, "
+                        f"gradient w.r.t. {len(shared)} strengths",
+            "forward_ms": f_ms, "reverse_ms": b_ms, "gradient_over_forward": (f_ms + b_ms) / f_ms,
+            "forward_tflops": flop / (f_ms * 1e-3) / 1e12, "survivors": int((lost == turns).sum())}
 
 
 def measured_traffic(args):

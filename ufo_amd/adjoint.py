@@ -42,27 +42,39 @@ ACCW = 16   # LDS accumulator slots per parameter (lane % ACCW)
 
 
 class _Grad:
-    """Gradient registers of one emitted segment: param column -> name."""
+    """Gradient registers of one emitted segment.  Coefficient adjoints are
+    summed per (parameter, derivative) pair — the slices of one magnet share
+    both — and scaled by the derivative once, at the flush."""
 
     def __init__(self):
-        self.regs: Dict[int, str] = {}
+        self.sums: Dict[tuple, str] = {}
 
-    def reg(self, i: int) -> str:
-        if i not in self.regs:
-            self.regs[i] = f"G{i}"
-        return self.regs[i]
+    def reg(self, i: int, d) -> str:
+        key = (i, d)
+        if key not in self.sums:
+            self.sums[key] = f"S{len(self.sums)}"
+        return self.sums[key]
+
+    def params(self):
+        return sorted({i for i, _ in self.sums})
 
 
-def _add_grad(env: Env, g: _Grad, coef: ir.Expr, adj: str, out: List[str]):
-    """G_i += adj * d coef / d theta_i for every shared parameter i in coef."""
+def _add_grad(env: Env, g: _Grad, coef: ir.Expr, adj, out: List[str]):
+    """Accumulate adj * d coef / d theta_i for every shared parameter i in
+    coef.  ``adj`` is a C expression, or a list of (factor, variable) pairs
+    summed with fused multiply-adds."""
     for i in sorted(ir.Expr.of(coef).params()):
         if i < env.shared_offset:
             continue
         d = env.f(ir.diff(coef, i))
         if is_zero(d):
             continue
-        r = g.reg(i)
-        out.append(f"{r} = fma({adj}, {env.val(d)}, {r});")
+        r = g.reg(i, d)
+        if isinstance(adj, list):
+            for a, b in adj:
+                out.append(f"{r} = fma({a}, {b}, {r});")
+        else:
+            out.append(f"{r} = {r} + {adj};")
 
 
 def emit_reverse(op, env: Env, out: List[str], g: _Grad):
@@ -73,7 +85,7 @@ def emit_reverse(op, env: Env, out: List[str], g: _Grad):
             return
         v = env.val(L)
         out.append(f"x = fma(-px, {v}, x); y = fma(-py, {v}, y);")
-        _add_grad(env, g, op.L, "fma(lx, px, ly * py)", out)
+        _add_grad(env, g, op.L, [("lx", "px"), ("ly", "py")], out)
         out.append(f"lpx = fma(lx, {v}, lpx); lpy = fma(ly, {v}, lpy);")
     elif isinstance(op, ir.ExactDrift):
         L = env.val(op.L)
@@ -93,7 +105,7 @@ def emit_reverse(op, env: Env, out: List[str], g: _Grad):
         if not is_zero(w):
             v = env.val(w)
             out.append(f"px = fma({v}, x, px);")
-            _add_grad(env, g, op.w, "(-lpx * x)", out)
+            _add_grad(env, g, op.w, [("-lpx", "x")], out)
             out.append(f"lx = fma({env.neg(w)}, lpx, lx);")
     elif isinstance(op, ir.Shift):
         dx, dy = f(op.dx), f(op.dy)
@@ -113,8 +125,8 @@ def emit_reverse(op, env: Env, out: List[str], g: _Grad):
             out.append(f"px = fma(-x, {env.val(t1)}, px);")
         if not is_zero(t2):
             out.append(f"py = fma(y, {env.val(t2)}, py);")
-        _add_grad(env, g, t1e, "(lpx * x)", out)
-        _add_grad(env, g, t2e, "(-lpy * y)", out)
+        _add_grad(env, g, t1e, [("lpx", "x")], out)
+        _add_grad(env, g, t2e, [("-lpy", "y")], out)
         if not is_zero(t1):
             out.append(f"lx = fma({env.val(t1)}, lpx, lx);")
         if not is_zero(t2):
@@ -140,8 +152,9 @@ def emit_reverse(op, env: Env, out: List[str], g: _Grad):
 
 
 def emit_kick_reverse(op: ir.Kick, env: Env, out: List[str], g: _Grad):
-    """Inverse kick px += Re w, py -= Im w; lambda_x -= a lpx - b lpy,
-    lambda_y += b lpx + a lpy with w' = a + i b; coefficient adjoints
+    """Inverse kick px += Re w, py -= Im w (the last Horner level fused into
+    px/py); lambda_x -= a lpx - b lpy, lambda_y += b lpx + a lpy with
+    w' = a + i b; coefficient adjoints
     dL/d kn_n = (-lpx Re z^n + lpy Im z^n) chi / n!,
     dL/d ks_n = (lpx Im z^n + lpy Re z^n) chi / n!  (chi = oodppo or 1)."""
     d = kick_coeffs(op, env)
@@ -151,54 +164,71 @@ def emit_kick_reverse(op: ir.Kick, env: Env, out: List[str], g: _Grad):
         if not is_zero(cr) or not is_zero(ci):
             top = i
     lines: List[str] = []
-    # w and w' at z (x, y unchanged by the kick)
+
+    def v(e):
+        return None if is_zero(e) else env.val(e)
+
+    dr = di = None
     if top >= 0:
-        wr = None if is_zero(d[top][0]) else env.val(d[top][0])
-        wi = None if is_zero(d[top][1]) else env.val(d[top][1])
+        wr, wi = v(d[top][0]), v(d[top][1])
         if top >= 1:
-            dr = None if is_zero(env.f(d[top][0] * float(top))) else env.val(env.f(d[top][0] * float(top)))
-            di = None if is_zero(env.f(d[top][1] * float(top))) else env.val(env.f(d[top][1] * float(top)))
+            dr, di = v(env.f(d[top][0] * float(top))), v(env.f(d[top][1] * float(top)))
         for lvl in range(top - 1, -1, -1):
-            cr = None if is_zero(d[lvl][0]) else env.val(d[lvl][0])
-            ci = None if is_zero(d[lvl][1]) else env.val(d[lvl][1])
             if lvl >= 1:
-                er = env.f(d[lvl][0] * float(lvl))
-                ei = env.f(d[lvl][1] * float(lvl))
-                er = None if is_zero(er) else env.val(er)
-                ei = None if is_zero(ei) else env.val(ei)
+                er, ei = v(env.f(d[lvl][0] * float(lvl))), v(env.f(d[lvl][1] * float(lvl)))
                 nr = codegen._chain([(dr, "x", False), (di, "y", True)], er)
                 ni = codegen._chain([(dr, "y", False), (di, "x", False)], ei)
                 a, b = env.new_tmp(), env.new_tmp()
                 lines.append(f"ufloat {a} = {nr or env.lit(0.0)}; ufloat {b} = {ni or env.lit(0.0)};")
                 dr, di = a, b
-            nr = codegen._chain([(wr, "x", False), (wi, "y", True)], cr)
-            ni = codegen._chain([(wr, "y", False), (wi, "x", False)], ci)
-            a, b = env.new_tmp(), env.new_tmp()
-            lines.append(f"ufloat {a} = {nr or env.lit(0.0)}; ufloat {b} = {ni or env.lit(0.0)};")
-            wr, wi = a, b
-        if wr is not None:
-            lines.append(f"px = px + {wr};")
-        if wi is not None:
-            lines.append(f"py = py - {wi};")
-    # coefficient adjoints: powers of z
-    params_here = any(ir.Expr.of(c).params() for c in list(op.kn) + list(op.ks))
-    if params_here:
-        zr, zi = env.lit(1.0), env.lit(0.0)
-        for m in range(n):
-            if m > 0:
+            if lvl == 0:
+                # px += Re(d0 + w z), py -= Im(d0 + w z), accumulated in place
+                c0r, c0i = v(d[0][0]), v(d[0][1])
+                px, py = "px", "py"
+                if wr is not None:
+                    px = _fma(wr, "x", px)
+                    py = _fma(_negs(wr), "y", py)
+                if wi is not None:
+                    px = _fma(_negs(wi), "y", px)
+                    py = _fma(_negs(wi), "x", py)
+                if c0r is not None:
+                    px = f"({px} + {c0r})"
+                if c0i is not None:
+                    py = f"({py} - {c0i})"
+                lines.append(f"px = {px}; py = {py};")
+            else:
+                cr, ci = v(d[lvl][0]), v(d[lvl][1])
+                nr = codegen._chain([(wr, "x", False), (wi, "y", True)], cr)
+                ni = codegen._chain([(wr, "y", False), (wi, "x", False)], ci)
                 a, b = env.new_tmp(), env.new_tmp()
-                lines.append(f"ufloat {a} = fma({zr}, x, -({zi}) * y); ufloat {b} = fma({zr}, y, ({zi}) * x);")
+                lines.append(f"ufloat {a} = {nr or env.lit(0.0)}; ufloat {b} = {ni or env.lit(0.0)};")
+                wr, wi = a, b
+        if top == 0:
+            c0r, c0i = v(d[0][0]), v(d[0][1])
+            if c0r is not None:
+                lines.append(f"px = px + {c0r};")
+            if c0i is not None:
+                lines.append(f"py = py - {c0i};")
+    # coefficient adjoints with the powers of z (z^0 = 1, z^1 = x + i y)
+    if any(ir.Expr.of(c).params() for c in list(op.kn) + list(op.ks)):
+        chi = ir.OODPPO if op.chroma else ir.E(1.0)
+        zr = zi = None
+        for m in range(n):
+            if m == 1:
+                zr, zi = "x", "y"
+            elif m >= 2:
+                a, b = env.new_tmp(), env.new_tmp()
+                lines.append(f"ufloat {a} = fma({zr}, x, -({zi} * y)); ufloat {b} = fma({zr}, y, {zi} * x);")
                 zr, zi = a, b
-            chi = ir.OODPPO if op.chroma else ir.E(1.0)
             scale = chi / float(math.factorial(m))
             if m < len(op.kn) and ir.Expr.of(op.kn[m]).params():
-                _add_grad(env, g, op.kn[m] * scale, f"fma(-lpx, {zr}, lpy * {zi})", lines)
+                adj = [("-lpx", env.lit(1.0))] if m == 0 else [("-lpx", zr), ("lpy", zi)]
+                _add_grad(env, g, op.kn[m] * scale, adj, lines)
             if m < len(op.ks) and ir.Expr.of(op.ks[m]).params():
-                _add_grad(env, g, op.ks[m] * scale, f"fma(lpx, {zi}, lpy * {zr})", lines)
+                adj = [("lpy", env.lit(1.0))] if m == 0 else [("lpx", zi), ("lpy", zr)]
+                _add_grad(env, g, op.ks[m] * scale, adj, lines)
     if top >= 1:
-        # lambda update with w' = dr + i di
-        lx = "lx"
-        ly = "ly"
+        lx, ly = "lx", "ly"
         if dr is not None:
             lx = _fma(_negs(dr), "lpx", lx)
             ly = _fma(dr, "lpy", ly)
@@ -253,12 +283,16 @@ def _reverse_layout(ops, env: Env, ntheta: int):
         body: List[str] = []
         for op in reversed(seg_ops):
             emit_reverse(op, env, body, g)
-        decl = [f"ufloat {r} = {env.lit(0.0)};" for r in g.regs.values()]
+        decl = [f"ufloat {r} = {env.lit(0.0)};" for r in g.sums.values()]
         flush = []
-        for i, r in sorted(g.regs.items()):
+        for i in g.params():
+            terms = [(r, env.val(dd)) for (ii, dd), r in g.sums.items() if ii == i]
+            acc_ = f"{terms[0][0]} * {terms[0][1]}"
+            for r, dv in terms[1:]:
+                acc_ = f"fma({r}, {dv}, {acc_})"
             j = i - env.shared_offset
             idx = f"({j} + {lv} * {stride})" if (lv and stride) else f"{j}"
-            flush.append(f"unsafeAtomicAdd(&acc[{idx} * {ACCW} + slot], {r});")
+            flush.append(f"unsafeAtomicAdd(&acc[{idx} * {ACCW} + slot], {acc_});")
         return ["{"] + ["    " + s for s in decl + body + flush] + ["}"]
 
     out: List[str] = []
