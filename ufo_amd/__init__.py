@@ -41,6 +41,9 @@ def __getattr__(name):
     if name in ("Optics",):
         from .optics import Optics
         return Optics
+    if name in ("BatchedAperture",):
+        from .batch import BatchedAperture
+        return BatchedAperture
     if name in ("AdjointTracker",):
         from .adjoint import AdjointTracker
         return AdjointTracker

@@ -1,0 +1,362 @@
+"""Batched dynamic aperture: many lattice variants (error seeds), each with its
+own particle count, in one persistent launch per block of turns.
+
+Layout and schedule:
+  * Per-seed lattice parameters (misalignments, multipole errors, strengths)
+    are lowered to a per-seed coefficient table ``ct[seed][slots]``: every
+    kick coefficient, shift or matrix entry that depends on the seed is
+    evaluated on the host from the IR expression, so the kernel reads final
+    coefficients with wave-uniform scalar loads and spends no VALU work on
+    them.  In a ring of repeated cells the table repeats per cell and the
+    cell loop indexes it with the repetition number.
+  * Particles are grouped by seed; each seed's alive particles form tiles of
+    64 (one wave), so a wave always works on one seed and its coefficients
+    are wave-uniform.
+  * ``ufo_bt_turns`` is persistent: each wave claims tiles (seed, tile) from
+    an atomic counter until the tile list of the block is exhausted, runs the
+    block's turns for its 64 particles and stores state and loss turns.
+  * Between blocks ``ufo_bt_compact`` (one workgroup per seed) compacts the
+    seed's alive particles and ``ufo_bt_scan`` rebuilds the tile offsets and
+    clears the counter (a kernel, not a memset), so the next block runs dense
+    tiles.
+  * The sequence (compact, scan, turns) x blocks is captured once into a
+    hipGraph and replayed by every ``run()``.
+  * The autotuner times candidate (turns per block, waves per SIMD) pairs on
+    the first blocks and keeps the fastest, cached per kernel and device.
+"""
+from __future__ import annotations
+
+import hashlib
+import json
+import os
+import time
+
+import numpy as np
+import torch
+
+from . import codegen, ir, runtime
+from .codegen import Env
+from .flags import DOUBLE_PRECISION, FIVED
+from .sim import Simulation, host_buffer
+
+TILE = 64
+
+
+def _turn_ops(line, flags, parameters):
+    return [op for piece in line.method(flags=flags, parameters=parameters) for op in piece]
+
+
+def batch_kernels(line, flags, coords, lattice_parameters, dp=0.0, block=256):
+    """Kernel source, table layout and facts of the batched tracker."""
+    coords = list(coords)
+    parameters = coords + list(lattice_parameters)
+    ops = _turn_ops(line, flags, parameters)
+    src = codegen._coord_sources(coords, flags, dp)
+    env = codegen._env_for(flags, src, None)
+    env.shared_offset = len(coords)
+    env.table = []
+    env.period_slots = {}
+    body, period = codegen._layout(ops, env)
+    head, p, reps, stride = env.last_period
+    F = len(env.table)
+    S = len(env.period_slots)
+    text = "\n".join(body).replace("@PB@", str(F)).replace("@PS@", str(max(S, 1)))
+    nslots = max(1, F + (reps * S if S else 0))
+    for e, _ in env.table:
+        if min(e.params()) < env.shared_offset:
+            raise ValueError("per-particle parameters are not supported by the batched tracker")
+    if any(i < len(coords) for op in ops for e in op.exprs() for i in e.params()):
+        raise ValueError("per-particle parameters other than the initial coordinates are not supported")
+    coord_cols = {c: i for i, c in enumerate(coords)}
+    for c in ("x", "px", "y", "py"):
+        if c not in coord_cols:
+            coord_cols[c] = None
+    one = env.lit(1.0)
+    zero = env.lit(0.0)
+    init = []
+    init.append(f"z = {zero};")
+    dpv = src["dp"]
+    if dpv[0] == "c":
+        init.append(f"dp = {env.lit(dpv[1])};")
+    else:
+        raise ValueError("the batched tracker runs at fixed dp (FIVED or no dp column)")
+    init.append(f"oodppo = {env.lit(env.consts['oodppo'])};")
+    init += env.inv_code
+    if env.params_used:
+        raise ValueError("unexpected per-particle parameter in the batched tracker")
+    pre = codegen._PRELUDE.format(ufloat="float" if env.fp32 else "double",
+                                  nan='__builtin_nanf("")' if env.fp32 else '__builtin_nan("")')
+    code = pre + f"""
+// one workgroup per seed: compact the seed's alive particles, count tiles
+extern "C" __global__ void __launch_bounds__(256)
+ufo_bt_compact(const int *__restrict__ lost, const int *__restrict__ seed_off, int *__restrict__ idx,
+               int *__restrict__ seed_cnt, int *__restrict__ seed_tiles, int turns)
+{{
+    __shared__ int base;
+    const int s = blockIdx.x;
+    const int b = seed_off[s], e = seed_off[s + 1];
+    if (threadIdx.x == 0) base = 0;
+    __syncthreads();
+    for (int i0 = b; i0 < e; i0 += 256) {{
+        const int i = i0 + threadIdx.x;
+        const bool alive = i < e && lost[i] >= turns;
+        unsigned long long m = __ballot(alive);
+        int lane = __lane_id();
+        int before = __popcll(m & ((1ull << lane) - 1ull));
+        int wbase = 0;
+        if (lane == 0) wbase = atomicAdd(&base, __popcll(m));
+        wbase = __shfl(wbase, 0);
+        if (alive) idx[b + wbase + before] = i;
+        __syncthreads();
+    }}
+    if (threadIdx.x == 0) {{
+        seed_cnt[s] = base;
+        seed_tiles[s] = (base + {TILE - 1}) / {TILE};
+    }}
+}}
+
+// tile offsets over seeds (exclusive scan), work counter cleared
+extern "C" __global__ void __launch_bounds__(64)
+ufo_bt_scan(const int *__restrict__ seed_tiles, int *__restrict__ tile_base, int *__restrict__ counter, int seeds)
+{{
+    if (threadIdx.x == 0) {{
+        int acc = 0;
+        for (int s = 0; s < seeds; ++s) {{ tile_base[s] = acc; acc += seed_tiles[s]; }}
+        tile_base[seeds] = acc;
+        counter[0] = 0;
+    }}
+}}
+
+extern "C" __global__ void __launch_bounds__({block})
+ufo_bt_turns(ufloat *__restrict__ state, int *__restrict__ lost, const int *__restrict__ idx,
+             const int *__restrict__ seed_off, const int *__restrict__ seed_cnt, const int *__restrict__ tile_base,
+             const ufloat *__restrict__ table, int *__restrict__ counter, int seeds, int nparticles,
+             int turn0, int nturns, int turns)
+{{
+    {codegen._decls(env)}
+    const int lane = __lane_id();
+    const size_t np = (size_t)nparticles;
+    const int ntiles = tile_base[seeds];
+    for (;;) {{
+        int w = 0;
+        if (lane == 0) w = atomicAdd(counter, 1);
+        w = __builtin_amdgcn_readfirstlane(__shfl(w, 0));
+        if (w >= ntiles) break;
+        int lo = 0, hi = seeds - 1;
+        while (lo < hi) {{                       // seed s with tile_base[s] <= w < tile_base[s+1]
+            int mid = (lo + hi + 1) >> 1;
+            if (tile_base[mid] <= w) lo = mid; else hi = mid - 1;
+        }}
+        const int s = __builtin_amdgcn_readfirstlane(lo);
+        const int t = w - tile_base[s];
+        const int k = t * {TILE} + lane;
+        if (k < seed_cnt[s]) {{
+        const int i = idx[seed_off[s] + k];
+        const ufloat *__restrict__ ct = table + (size_t)s * {nslots};
+{chr(10).join('        ' + x for x in init)}
+        x = state[i]; px = state[np + i]; y = state[2 * np + i]; py = state[3 * np + i];
+        int turn = turn0;
+        const int tend = (turn0 + nturns < turns) ? turn0 + nturns : turns;
+        bool gone = false;
+        for (; turn < tend; ++turn) {{
+{chr(10).join('            ' + x_ for x_ in text.split(chr(10)))}
+            gone = fabs(x) > {one} || fabs(y) > {one} || isnan(x);
+            if (gone) break;
+        }}
+        state[i] = x; state[np + i] = px; state[2 * np + i] = y; state[3 * np + i] = py;
+        if (gone) lost[i] = turn;
+        }}
+    }}
+}}
+"""
+    layout = {"flat": [e for e, _ in env.table], "period": sorted(env.period_slots, key=env.period_slots.get),
+              "reps": reps, "stride": stride, "F": F, "S": S, "nslots": nslots, "offset": len(coords)}
+    k = codegen.Kernel(name="ufo_bt_turns", src=code, kind="batch", fp32=env.fp32,
+                       flops_per_turn=codegen._count(ops, env), elements=codegen.element_count(line), block=block,
+                       period=period)
+    return k, layout
+
+
+def build_table(layout, seed_params):
+    """Per-seed coefficient table [seeds, nslots] from the per-seed values of
+    the lattice parameters [seeds, P] (parameter column offset: the
+    coordinates come first)."""
+    seeds = seed_params.shape[0]
+    off = layout["offset"]
+    ct = np.zeros((seeds, layout["nslots"]))
+    for s in range(seeds):
+        full = np.concatenate([np.zeros(off), seed_params[s]])
+        for k, e in enumerate(layout["flat"]):
+            ct[s, k] = ir.evaluate(e, full)
+        for j, e in enumerate(layout["period"]):
+            for r in range(layout["reps"]):
+                ct[s, layout["F"] + j + r * layout["S"]] = ir.evaluate(
+                    ir.shift_params(e, r * layout["stride"], off), full)
+    return ct
+
+
+def _tune_cache():
+    d = os.environ.get("UFO_CACHE_DIR") or os.path.join(os.path.expanduser("~"), ".cache", "ufo_amd")
+    try:
+        os.makedirs(d, exist_ok=True)
+        return os.path.join(d, "tune.json")
+    except OSError:
+        return os.path.join("/tmp", f"ufo_amd_tune_{os.getuid()}.json")
+
+
+class BatchedAperture(Simulation):
+    """StableAperture for ``seeds`` lattice variants in one launch sequence.
+
+    ``lattice_parameters``: (label-or-index, name) tuples, as the reference's
+    parameters; ``seed_values`` [seeds, len(lattice_parameters)] gives each
+    seed's values; ``counts`` the number of particles of each seed.  Rows of
+    ``parameters`` [sum(counts), len(coords)] are grouped by seed, seed 0
+    first.  After ``run()``, ``lost`` holds the reference's StableAperture
+    result per particle.  ``block_turns`` turns run per launch; None lets the
+    autotuner choose.
+    """
+
+    def __init__(self, line, flags, turns, lattice_parameters, seed_values, counts, coords=("x", "y"), dp=0.0,
+                 block_turns=None, context=None, options=None):
+        self._init_common(flags, context, options)
+        self.turns = int(turns)
+        self.coords = list(coords)
+        self.lattice_parameters = list(lattice_parameters)
+        self.counts = np.asarray(counts, dtype=np.int64)
+        self.seeds = len(self.counts)
+        self.seed_values = np.asarray(seed_values, dtype=np.float64).reshape(self.seeds, -1)
+        self.kernel, self.layout = batch_kernels(line, flags, self.coords, self.lattice_parameters, dp)
+        self.table = build_table(self.layout, self.seed_values)
+        self.particles = int(self.counts.sum())
+        self.offsets = np.concatenate([[0], np.cumsum(self.counts)]).astype(np.int32)
+        self._pool_host = host_buffer((self.particles, max(1, len(self.coords))), self.dtype)
+        self.parameters = self._pool_host.numpy()
+        self._lost_host = host_buffer((self.particles,), torch.int32)
+        self.lost = self._lost_host.numpy().view(np.uint32)
+        self.block_turns = block_turns
+        self.waves_per_simd = None
+        self._dev = None
+        self._graphs = {}
+        self._stream = None
+
+    def _buffers(self):
+        if self._dev is None:
+            d, n, S = self.device, self.particles, self.seeds
+            self._dev = {
+                "state": torch.zeros((4, max(1, n)), dtype=self.dtype, device=d),
+                "lost": torch.empty((max(1, n),), dtype=torch.int32, device=d),
+                "idx": torch.empty((max(1, n),), dtype=torch.int32, device=d),
+                "seed_off": torch.tensor(self.offsets, dtype=torch.int32, device=d),
+                "seed_cnt": torch.empty((S,), dtype=torch.int32, device=d),
+                "seed_tiles": torch.empty((S,), dtype=torch.int32, device=d),
+                "tile_base": torch.empty((S + 1,), dtype=torch.int32, device=d),
+                "counter": torch.zeros((1,), dtype=torch.int32, device=d),
+                "table": torch.tensor(self.table, dtype=self.dtype, device=d),
+                "pool": torch.empty(self._pool_host.shape, dtype=self.dtype, device=d),
+            }
+        return self._dev
+
+    def _reset(self):
+        """State from the pool, everybody alive."""
+        b = self._buffers()
+        b["pool"].copy_(self._pool_host, non_blocking=True)
+        st = b["state"]
+        st.zero_()
+        for c in ("x", "px", "y", "py"):
+            if c in self.coords:
+                st[("x", "px", "y", "py").index(c), :self.particles] = b["pool"][:, self.coords.index(c)]
+        b["lost"].fill_(self.turns)
+
+    def _grid(self, waves_per_simd):
+        lanes = self.ctx.device.compute_units * 4 * waves_per_simd * 64
+        block = self.kernel.block
+        return max(1, lanes // block)
+
+    def _enqueue_block(self, turn0, nturns, waves_per_simd):
+        b = self._buffers()
+        mod = self.module()
+        mod.launch("ufo_bt_compact", self.seeds, 256, [b["lost"], b["seed_off"], b["idx"], b["seed_cnt"],
+                                                         b["seed_tiles"], self.turns], stream=self._stream.cuda_stream)
+        mod.launch("ufo_bt_scan", 1, 64, [b["seed_tiles"], b["tile_base"], b["counter"], self.seeds],
+                   stream=self._stream.cuda_stream)
+        mod.launch("ufo_bt_turns", self._grid(waves_per_simd), self.kernel.block,
+                   [b["state"], b["lost"], b["idx"], b["seed_off"], b["seed_cnt"], b["tile_base"], b["table"],
+                    b["counter"], self.seeds, self.particles, turn0, nturns, self.turns],
+                   stream=self._stream.cuda_stream)
+
+    def _graph(self, block_turns, waves_per_simd):
+        key = (block_turns, waves_per_simd)
+        g = self._graphs.get(key)
+        if g is None:
+            def seq():
+                for t0 in range(0, self.turns, block_turns):
+                    self._enqueue_block(t0, block_turns, waves_per_simd)
+            g = runtime.capture(self._stream.cuda_stream, seq)
+            self._graphs[key] = g
+        return g
+
+    def tune(self, candidates=((32, 8), (64, 8), (128, 8), (64, 4)), sample_turns=256):
+        """Pick (turns per block, waves per SIMD) by timing the first
+        ``sample_turns`` turns of each candidate; cached per kernel and device."""
+        key = hashlib.sha1((self.kernel.src + self.ctx.device.arch).encode()).hexdigest()[:16] + \
+            f"-{self.particles}-{self.turns}"
+        path = _tune_cache()
+        try:
+            cache = json.load(open(path))
+        except (OSError, ValueError):
+            cache = {}
+        if key in cache:
+            self.block_turns, self.waves_per_simd = cache[key]
+            return self.block_turns, self.waves_per_simd
+        self._stream = self._stream or torch.cuda.Stream(self.device)
+        best = None
+        for bt, wps in candidates:
+            self._reset()
+            torch.cuda.synchronize()
+            with torch.cuda.stream(self._stream):
+                t = time.perf_counter()
+                for t0 in range(0, min(sample_turns, self.turns), bt):
+                    self._enqueue_block(t0, bt, wps)
+                self._stream.synchronize()
+                dt = time.perf_counter() - t
+            if best is None or dt < best[0]:
+                best = (dt, bt, wps)
+        self.block_turns, self.waves_per_simd = best[1], best[2]
+        cache[key] = [self.block_turns, self.waves_per_simd]
+        try:
+            json.dump(cache, open(path, "w"))
+        except OSError:
+            pass
+        return self.block_turns, self.waves_per_simd
+
+    def launch(self):
+        """Enqueue the whole run (the captured graph) on this object's stream,
+        from the device pool."""
+        if self.block_turns is None or self.waves_per_simd is None:
+            if self.block_turns is None:
+                self.tune()
+            else:
+                self.waves_per_simd = 8
+        self._stream = self._stream or torch.cuda.Stream(self.device)
+        g = self._graph(self.block_turns, self.waves_per_simd)
+        b = self._buffers()
+        with torch.cuda.stream(self._stream):
+            st = b["state"]
+            st.zero_()
+            for c in ("x", "px", "y", "py"):
+                if c in self.coords:
+                    st[("x", "px", "y", "py").index(c), :self.particles] = b["pool"][:, self.coords.index(c)]
+            b["lost"].fill_(self.turns)
+            g.launch(self._stream.cuda_stream)
+
+    def run(self, threads=1):
+        b = self._buffers()
+        self._stream = self._stream or torch.cuda.Stream(self.device)
+        if self.block_turns is None:
+            self._reset()
+            self.tune()
+        b["pool"].copy_(self._pool_host)
+        torch.cuda.synchronize()
+        self.launch()
+        self._stream.synchronize()
+        self._lost_host.copy_(b["lost"][:self.particles])

@@ -71,6 +71,9 @@ class Env:
     tmp: int = 0
     param_mem: bool = False          # parameters read from the pool row at use
     shared_offset: int = 1 << 30     # parameter columns >= this come from the shared vector th[]
+    table: Optional[list] = None     # coefficient-table mode: [(expr, in_period)] slots of ct[]
+    period_slots: Optional[dict] = None
+    last_period: tuple = (0, 0, 0, 0)
     stride: Optional[tuple] = None   # (loop variable, parameter stride) inside a strided period loop
 
     def f(self, e) -> Expr:
@@ -114,6 +117,9 @@ class Env:
         not depend on the turn are hoisted into per-particle invariants."""
         e = self.f(e)
         k = e.t[0]
+        if self.table is not None and k != "c" and e.params() and not (e.uses_var("oodppo") or e.uses_var("dp")) \
+                and not e.has_raw() and min(e.params()) >= self.shared_offset:
+            return self.table_slot(e)
         if k in ("c", "v"):
             return self.raw(e)
         if k == "p":
@@ -137,6 +143,20 @@ class Env:
             self.invariants[e] = name
             self.inv_code.append(f"{name} = {text};")
         return name
+
+    def table_slot(self, e) -> str:
+        """Slot of a per-seed coefficient in the table ct[]; inside a strided
+        period loop the slot repeats per repetition."""
+        if self.stride is not None and self.stride[1] != 0:
+            slots = self.period_slots
+            if e not in slots:
+                slots[e] = len(slots)
+            return f"ct[@PB@ + {slots[e]} + {self.stride[0]} * @PS@]"
+        for k, (ex, _) in enumerate(self.table):
+            if ex == e:
+                return f"ct[{k}]"
+        self.table.append((e, None))
+        return f"ct[{len(self.table) - 1}]"
 
     def neg(self, e) -> str:
         e = self.f(e)
@@ -542,6 +562,7 @@ def _layout(body_ops, env, observe=None, loop_var="rep", emit=None):
     forward pass)."""
     emit = emit or emit_op
     head, p, reps, stride = find_period(body_ops, True) if len(body_ops) > 8 else (0, 0, 0, 0)
+    env.last_period = (head, p, reps, stride)
     out: List[str] = []
     if reps >= 2:
         for op in body_ops[:head]:

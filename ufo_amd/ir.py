@@ -184,6 +184,40 @@ def diff(e: Expr, i: int) -> Expr:
     raise ValueError(f"cannot differentiate {e!r}")
 
 
+def evaluate(e: Expr, params, env=None) -> float:
+    """Value of ``e`` in FP64 on the host with parameter column i bound to
+    ``params[i]`` and kernel variables to ``env``."""
+    t = Expr.of(e).t
+    k = t[0]
+    if k == "c":
+        return t[1]
+    if k == "p":
+        return float(params[t[1]])
+    if k == "v":
+        return float(env[t[1]])
+    if k == "r":
+        raise ValueError("raw C text cannot be evaluated on the host")
+    if k == "neg":
+        return -evaluate(t[1], params, env)
+    if k == "f":
+        return _LIBM[t[1]](evaluate(t[2], params, env))
+    if k == "?":
+        return evaluate(t[2] if evaluate(t[1], params, env) != 0.0 else t[3], params, env)
+    a = evaluate(t[1], params, env)
+    b = evaluate(t[2], params, env)
+    if k == ">":
+        return float(a > b)
+    if k == "<":
+        return float(a < b)
+    return _binop(k, a, b, False)
+
+
+def shift_params(e: Expr, offset: int, start: int = 0) -> Expr:
+    """Copy of ``e`` with parameter columns >= start moved by ``offset``."""
+    return map_leaves(Expr.of(e), lambda l: Expr(("p", l.t[1] + offset)) if l.t[0] == "p" and l.t[1] >= start
+                      else l)
+
+
 def select(cond, a, b) -> Expr:
     """``cond ? a : b`` (cond is a comparison expression)."""
     return Expr(("?", Expr.of(cond), Expr.of(a), Expr.of(b)))

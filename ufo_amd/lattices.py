@@ -102,3 +102,54 @@ def gaussian_bunch(n, seed=0, sx=1e-3, spx=1e-4, sy=1e-3, spy=1e-4, sdp=1e-3):
     for j, s in enumerate((sx, spx, sy, spy, sdp)):
         out[:, j] = rng.normal(0.0, s, n)
     return out
+
+
+ERROR_ORDERS = (2, 3, 4, 5)          # b3..b6 / a3..a6 in 1-based (European) naming
+REFERENCE_RADIUS = 0.01              # m
+
+
+def dba_error_model(periods=4, seeds=256, seed0=1000, sigma_align=50e-6, sigma_rel=1e-4,
+                    count_range=(4096, 32768)):
+    """Error seeds of the ``periods``-period DBA ring.
+
+    Every quadrupole and sextupole gets per seed: dx, dy ~ N(0, sigma_align),
+    and normal/skew multipole errors dk_n, dk_ns (n = 2..5, i.e. b3..b6 and
+    a3..a6) with relative strength b_n ~ N(0, sigma_rel) of the main field at
+    the reference radius R = 10 mm: dk_n = b_n K_m R^(m-n) n!/m!, K_m the
+    magnet's main strength (k1, m = 1; k2, m = 2) in the same per-length units.
+    Seed s draws from numpy.random.default_rng(seed0 + s); its particle count
+    is uniform in ``count_range``.
+
+    Returns (ring, lattice_parameters, seed_values [seeds, P], counts [seeds]).
+    """
+    import numpy as np
+    ring = dba_ring(periods, distinct=True)
+    magnets = [e for e in ring.flatten() if isinstance(e, (Quadrupole, Sextupole))]
+    names = ["dx", "dy"] + [f"dk{n}" for n in ERROR_ORDERS] + [f"dk{n}s" for n in ERROR_ORDERS]
+    params = [(m.label, nm) for m in magnets for nm in names]
+    for m in magnets:       # make the dkN / dkNs attributes addressable
+        m.dkn = [0.0] * (max(ERROR_ORDERS) + 1)
+        m.dks = [0.0] * (max(ERROR_ORDERS) + 1)
+    values = np.zeros((seeds, len(params)))
+    counts = np.zeros(seeds, dtype=np.int64)
+    for s in range(seeds):
+        rng = np.random.default_rng(seed0 + s)
+        row = []
+        for m in magnets:
+            order, K = (1, m.k1) if isinstance(m, Quadrupole) else (2, m.k2)
+            row += list(rng.normal(0.0, sigma_align, 2))
+            b = rng.normal(0.0, sigma_rel, 2 * len(ERROR_ORDERS))
+            for j, n in enumerate(list(ERROR_ORDERS) * 2):
+                row.append(b[j] * K * REFERENCE_RADIUS ** (order - n) * math.factorial(n) / math.factorial(order))
+        values[s] = row
+        counts[s] = rng.integers(count_range[0], count_range[1] + 1)
+    return ring, params, values, counts
+
+
+def seed_grid(n, xlim=0.02, ymax=0.008):
+    """First n points (x fastest) of the smallest ~5:1 grid over the DA
+    rectangle holding n points: the same grid law for every particle count."""
+    import numpy as np
+    nx = int(math.ceil(math.sqrt(5.0 * n)))
+    ny = int(math.ceil(n / nx))
+    return da_grid(nx, ny, xlim, ymax)[:n]
