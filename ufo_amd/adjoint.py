@@ -64,7 +64,7 @@ def _add_grad(env: Env, g: _Grad, coef: ir.Expr, adj, out: List[str]):
     coef.  ``adj`` is a C expression, or a list of (factor, variable) pairs
     summed with fused multiply-adds."""
     for i in sorted(ir.Expr.of(coef).params()):
-        if i < env.shared_offset:
+        if i < env.shared_offset or (env.diff_cols is not None and i not in env.diff_cols):
             continue
         d = env.f(ir.diff(coef, i))
         if is_zero(d):
@@ -157,6 +157,8 @@ def emit_kick_reverse(op: ir.Kick, env: Env, out: List[str], g: _Grad):
     w' = a + i b; coefficient adjoints
     dL/d kn_n = (-lpx Re z^n + lpy Im z^n) chi / n!,
     dL/d ks_n = (lpx Im z^n + lpy Re z^n) chi / n!  (chi = oodppo or 1)."""
+    if codegen.chroma_factored(op, env):
+        return emit_kick_reverse_factored(ir.Kick(op.kn, op.ks, False), env, out, g)
     d = kick_coeffs(op, env)
     n = len(d)
     top = -1
@@ -238,6 +240,46 @@ def emit_kick_reverse(op: ir.Kick, env: Env, out: List[str], g: _Grad):
         lines.append(f"lx = {lx}; ly = {ly};")
     if lines:
         out.append("{ " + " ".join(lines) + " }")
+
+
+def emit_kick_reverse_factored(op: ir.Kick, env: Env, out: List[str], g: _Grad):
+    """Reverse of a kick whose 1/(1+dp) factor is applied to the whole sum
+    (coefficient-table mode): px += oodppo Re w, py -= oodppo Im w, and the
+    lambda update and coefficient adjoints use oodppo * (lpx, lpy)."""
+    lines: List[str] = []
+    wr, wi = codegen.kick_sum(op, env, lines)
+    if wr is not None:
+        lines.append(f"px = fma(oodppo, {wr}, px);")
+    if wi is not None:
+        lines.append(f"py = fma(-oodppo, {wi}, py);")
+    lines.append("ufloat slx = oodppo * lpx, sly = oodppo * lpy;")
+    if any(ir.Expr.of(c).params() for c in list(op.kn) + list(op.ks)):
+        zr = zi = None
+        for m in range(max(len(op.kn), len(op.ks))):
+            if m == 1:
+                zr, zi = "x", "y"
+            elif m >= 2:
+                a, b = env.new_tmp(), env.new_tmp()
+                lines.append(f"ufloat {a} = fma({zr}, x, -({zi} * y)); ufloat {b} = fma({zr}, y, {zi} * x);")
+                zr, zi = a, b
+            scale = 1.0 / float(math.factorial(m))
+            if m < len(op.kn) and ir.Expr.of(op.kn[m]).params():
+                adj = [("-slx", env.lit(1.0))] if m == 0 else [("-slx", zr), ("sly", zi)]
+                _add_grad(env, g, op.kn[m] * scale, adj, lines)
+            if m < len(op.ks) and ir.Expr.of(op.ks[m]).params():
+                adj = [("sly", env.lit(1.0))] if m == 0 else [("slx", zi), ("sly", zr)]
+                _add_grad(env, g, op.ks[m] * scale, adj, lines)
+    dr, di = codegen.kick_derivative(op, env, lines)
+    lx, ly = "lx", "ly"
+    if dr is not None:
+        lx = _fma(_negs(dr), "slx", lx)
+        ly = _fma(dr, "sly", ly)
+    if di is not None:
+        lx = _fma(di, "sly", lx)
+        ly = _fma(di, "slx", ly)
+    if (lx, ly) != ("lx", "ly"):
+        lines.append(f"lx = {lx}; ly = {ly};")
+    out.append("{ " + " ".join(lines) + " }")
 
 
 def emit_thick_reverse(op, env: Env, out: List[str], g: _Grad):

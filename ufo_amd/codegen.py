@@ -74,6 +74,8 @@ class Env:
     table: Optional[list] = None     # coefficient-table mode: [(expr, in_period)] slots of ct[]
     period_slots: Optional[dict] = None
     last_period: tuple = (0, 0, 0, 0)
+    diff_cols: Optional[list] = None  # parameter columns with gradients (None: every shared one)
+    no_loop: bool = False            # emit the turn flat (no period loop)
     stride: Optional[tuple] = None   # (loop variable, parameter stride) inside a strided period loop
 
     def f(self, e) -> Expr:
@@ -253,9 +255,19 @@ def kick_coeffs(op: ir.Kick, env: Env):
     return out
 
 
+def chroma_factored(op, env: Env) -> bool:
+    """In coefficient-table mode a per-particle 1/(1+dp) cannot be folded
+    into the per-seed coefficients: the kick is computed without it and
+    scaled as a whole (every coefficient carries the same factor)."""
+    return (env.table is not None and isinstance(op, ir.Kick) and op.chroma
+            and not env.f(ir.OODPPO).is_const())
+
+
 def emit_kick(op: ir.Kick, env: Env, out: List[str]):
     """w = sum d_n z^n by Horner, px -= Re w, py += Im w; the last level
     accumulates straight into px/py."""
+    if chroma_factored(op, env):
+        return emit_kick_factored(ir.Kick(op.kn, op.ks, False), env, out)
     d = kick_coeffs(op, env)
     top = -1
     for i, (cr, ci) in enumerate(d):
@@ -306,6 +318,65 @@ def emit_kick(op: ir.Kick, env: Env, out: List[str]):
     if py != "py":
         lines.append(f"py = {py};")
     out.append("{ " + " ".join(lines) + " }")
+
+
+def kick_sum(op: ir.Kick, env: Env, lines: List[str]):
+    """(Re, Im) C names of w = sum d_n z^n (None for a zero part)."""
+    d = kick_coeffs(op, env)
+    top = -1
+    for i, (cr, ci) in enumerate(d):
+        if not is_zero(cr) or not is_zero(ci):
+            top = i
+    if top < 0:
+        return None, None
+    wr = None if is_zero(d[top][0]) else env.val(d[top][0])
+    wi = None if is_zero(d[top][1]) else env.val(d[top][1])
+    for lvl in range(top - 1, -1, -1):
+        dr = None if is_zero(d[lvl][0]) else env.val(d[lvl][0])
+        di = None if is_zero(d[lvl][1]) else env.val(d[lvl][1])
+        r = _chain([(wr, "x", False), (wi, "y", True)], dr)
+        i = _chain([(wr, "y", False), (wi, "x", False)], di)
+        tr, ti = env.new_tmp(), env.new_tmp()
+        lines.append(f"ufloat {tr} = {r if r else env.lit(0.0)};")
+        lines.append(f"ufloat {ti} = {i if i else env.lit(0.0)};")
+        wr, wi = tr, ti
+    return wr, wi
+
+
+def kick_derivative(op: ir.Kick, env: Env, lines: List[str]):
+    """(Re, Im) C names of w'(z) = sum n d_n z^(n-1) (None for zero parts)."""
+    d = kick_coeffs(op, env)
+    top = -1
+    for i, (cr, ci) in enumerate(d):
+        if not is_zero(cr) or not is_zero(ci):
+            top = i
+    if top < 1:
+        return None, None
+    e = [(env.f(d[n + 1][0] * float(n + 1)), env.f(d[n + 1][1] * float(n + 1))) for n in range(top)]
+    wr = None if is_zero(e[-1][0]) else env.val(e[-1][0])
+    wi = None if is_zero(e[-1][1]) else env.val(e[-1][1])
+    for lvl in range(top - 2, -1, -1):
+        er = None if is_zero(e[lvl][0]) else env.val(e[lvl][0])
+        ei = None if is_zero(e[lvl][1]) else env.val(e[lvl][1])
+        r = _chain([(wr, "x", False), (wi, "y", True)], er)
+        i = _chain([(wr, "y", False), (wi, "x", False)], ei)
+        tr, ti = env.new_tmp(), env.new_tmp()
+        lines.append(f"ufloat {tr} = {r if r else env.lit(0.0)};")
+        lines.append(f"ufloat {ti} = {i if i else env.lit(0.0)};")
+        wr, wi = tr, ti
+    return wr, wi
+
+
+def emit_kick_factored(op: ir.Kick, env: Env, out: List[str]):
+    """px -= oodppo Re w, py += oodppo Im w with the factor outside the sum."""
+    lines: List[str] = []
+    wr, wi = kick_sum(op, env, lines)
+    if wr is not None:
+        lines.append(f"px = fma(-oodppo, {wr}, px);")
+    if wi is not None:
+        lines.append(f"py = fma(oodppo, {wi}, py);")
+    if lines:
+        out.append("{ " + " ".join(lines) + " }")
 
 
 def _negs(s: str) -> str:
@@ -561,7 +632,7 @@ def _layout(body_ops, env, observe=None, loop_var="rep", emit=None):
     one.  ``emit(op, env, out, observe)`` writes one op (default: the
     forward pass)."""
     emit = emit or emit_op
-    head, p, reps, stride = find_period(body_ops, True) if len(body_ops) > 8 else (0, 0, 0, 0)
+    head, p, reps, stride = find_period(body_ops, True) if len(body_ops) > 8 and not env.no_loop else (0, 0, 0, 0)
     env.last_period = (head, p, reps, stride)
     out: List[str] = []
     if reps >= 2:

@@ -153,3 +153,30 @@ def seed_grid(n, xlim=0.02, ymax=0.008):
     nx = int(math.ceil(math.sqrt(5.0 * n)))
     ny = int(math.ceil(n / nx))
     return da_grid(nx, ny, xlim, ymax)[:n]
+
+
+def dba_family_model(seeds=64, seed0=1000, **kw):
+    """The 4-period error model of ``dba_error_model`` with 8 sextupole
+    families: the ring's 4 sextupole families per period, split by period
+    parity (periods 0 and 2 form family set A, periods 1 and 3 set B).  The
+    family strengths k2 are per-seed parameter columns (equal for every seed)
+    bound by label; the per-magnet errors stay bound by flat element index.
+
+    Returns (ring, lattice_parameters, seed_values, counts, family_parameters).
+    """
+    import numpy as np
+    ring_e, prm_e, vals_e, counts = dba_error_model(periods=4, seeds=seeds, seed0=seed0, **kw)
+    periods = [dba_period(f"{'AB'[r % 2]}") for r in range(4)]
+    ring = Line("RING", periods + [Aperture("AP", window="(fabs(x) < 0.03 && fabs(y) < 0.03)")])
+    flat_e = ring_e.flatten()
+    flat = ring.flatten()
+    for e in flat:
+        if isinstance(e, (Quadrupole, Sextupole)):
+            e.dkn = [0.0] * (max(ERROR_ORDERS) + 1)
+            e.dks = [0.0] * (max(ERROR_ORDERS) + 1)
+    index = {e.label: i for i, e in enumerate(flat_e)}
+    prm = [(index[label], name) for label, name in prm_e]      # errors by flat index
+    fam = [(f"{n}{t}", "k2") for t in "AB" for n in ("SD1", "SF1", "SF2", "SD2")]
+    fam_vals = [DBA_SEXTS[n] for t in "AB" for n in ("SD1", "SF1", "SF2", "SD2")]
+    values = np.concatenate([vals_e, np.tile(fam_vals, (seeds, 1))], axis=1)
+    return ring, prm + fam, values, counts, fam
