@@ -44,6 +44,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-seed-adjoint", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="MiB copied by one device-to-device copy before the timed region "
                          "(reference kernel for HBM traffic counters)")
@@ -144,6 +145,8 @@ def main():
         out["adjoint"] = adjoint_timing()
     if not args.no_batched:
         out["batched"] = batched_timing()
+    if not args.no_seed_adjoint:
+        out["seed_adjoint"] = seed_adjoint_timing()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(ring, flags, pool, args.turns, E, args.cpu_seconds)
     print(json.dumps(out))
@@ -185,6 +188,67 @@ def batched_timing(seeds=256, turns=2048, reps=2):
             "waves_per_simd": ba.waves_per_simd}
 
 
+def seed_adjoint_timing(seeds=64, per_seed=8192, turns=1024, every=32):
+    """Batched sextupole-optimisation step: the 4-period error model, 64
+    seeds x 8192 particles (Gaussian sigma_x = 4 mm, sigma_y = 2 mm,
+    sigma_px,py = 0.1 mrad, sigma_delta = 2e-3) launched around each seed's
+    closed orbit (Newton on the tangent kernel, tolerance 1e-13 m),
+    1024 turns, per-seed gradient of the per-seed amplitude/survival loss with
+    respect to the 8 sextupole family strengths."""
+    import numpy as np
+    import torch
+    import ufo_amd as ufo
+    from ufo_amd import lattices
+    from ufo_amd.batch_adjoint import BatchedAdjoint, SeedClosedOrbit
+    flags = ufo.KICK | ufo.DOUBLE_PRECISION
+    ring, prm, vals, _, fam = lattices.dba_family_model(seeds=seeds)
+    counts = np.full(seeds, per_seed)
+    t = time.perf_counter()
+    co = SeedClosedOrbit(ring, flags, prm, vals, tol=1e-13)
+    co.module()
+    ba = BatchedAdjoint(ring, flags, turns, ["x", "px", "y", "py", "dp"], prm, vals, counts, fam, every=every)
+    ba._buffers()
+    compile_s = time.perf_counter() - t
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    orbits = co.run()                       # copies the orbits to the host: synchronous
+    co_ms = 1e3 * (time.perf_counter() - t)
+    rng = np.random.default_rng(4)
+    n = seeds * per_seed
+    x0 = np.stack([rng.normal(0, 4e-3, n), rng.normal(0, 1e-4, n), rng.normal(0, 2e-3, n),
+                   rng.normal(0, 1e-4, n), rng.normal(0, 2e-3, n)], 1)
+    seed_of = np.repeat(np.arange(seeds), per_seed)
+    x0[:, :4] += orbits[seed_of]
+    x0 = torch.tensor(x0, device=ba.device)
+    seg = torch.tensor(seed_of, device=ba.device)
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+    ev[0].record(stream)
+    xf, lost = ba.forward(x0)
+    ev[1].record(stream)
+    xf_t = xf.clone().requires_grad_(True)
+    alive = torch.isfinite(xf_t[:, 0])
+    r2 = torch.where(alive, xf_t[:, 0] ** 2 + xf_t[:, 2] ** 2, torch.zeros_like(xf_t[:, 0]))
+    pen = torch.where(alive, torch.nn.functional.softplus(20.0 * (r2 / 0.03 ** 2 - 0.25)) / 20.0,
+                      torch.ones_like(r2))
+    per = torch.zeros(seeds, dtype=torch.float64, device=ba.device).index_add_(0, seg, r2 + 1e-5 * pen) / per_seed
+    per.sum().backward()
+    ev[2].record(stream)
+    _, gseed = ba.backward(xf_t.grad)
+    ev[3].record(stream)
+    torch.cuda.synchronize()
+    f_ms, b_ms = ev[0].elapsed_time(ev[1]), ev[2].elapsed_time(ev[3])
+    return {"workload": f"{seeds} seeds x {per_seed} particles, 4-period DBA with errors, 6D, {turns} turns, "
+                        f"gradient w.r.t. {len(fam)} sextupole families per seed",
+            "closed_orbit_ms": round(co_ms, 2), "newton_steps_max": int(co.steps.max()),
+            "seeds_converged": int(co.converged.sum()),
+            "forward_ms": round(f_ms, 2), "reverse_ms": round(b_ms, 2),
+            "gradient_over_forward": round((f_ms + b_ms) / f_ms, 3),
+            "survivors": int((lost == turns).sum().item()), "particles": n,
+            "hiprtc_compile_s": round(compile_s, 1), "gradient_finite": bool(torch.isfinite(gseed).all().item())}
+
+
 def adjoint_timing(cells=48, particles=65536, turns=512, every=32, reps=3):
     """Gradient of the amplitude loss with respect to all 96 quadrupole k1 and
     96 sextupole k2 of a 48-cell FODO ring (65 536 particles, sigma_x =
@@ -223,9 +287,27 @@ def adjoint_timing(cells=48, particles=65536, turns=512, every=32, reps=3):
     f_ms, b_ms = float(np.mean(fw)), float(np.mean(bw))
     pt = int(torch.clamp(lost.to(torch.int64) + 1, max=turns).sum())
     flop = pt * tr.kfwd.flops_per_turn
+    # one plain forward pass of the same ring and bunch: the StableAperture
+    # kernel with the strengths compiled in (they equal theta)
+    sa = ufo.StableAperture(ring, flags=ufo.KICK | ufo.DOUBLE_PRECISION, turns=turns, particles=particles,
+                            parameters=["x", "px", "y", "py", "dp"])
+    sa.parameters[:] = bunch
+    sa.module()
+    sa.pool_device.copy_(sa._pool_host)
+    pf = []
+    for k in range(reps + 1):
+        e = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+        e[0].record(stream)
+        sa.launch()
+        e[1].record(stream)
+        torch.cuda.synchronize()
+        if k:
+            pf.append(e[0].elapsed_time(e[1]))
+    p_ms = float(np.mean(pf))
     return {"workload": f"{cells}-cell FODO, {particles} particles, {turns} turns, checkpoints every {every}, "
                         f"gradient w.r.t. {len(shared)} strengths",
             "forward_ms": f_ms, "reverse_ms": b_ms, "gradient_over_forward": (f_ms + b_ms) / f_ms,
+            "plain_forward_ms": p_ms, "gradient_over_plain_forward": (f_ms + b_ms) / p_ms,
             "forward_tflops": flop / (f_ms * 1e-3) / 1e12, "survivors": int((lost == turns).sum())}
 
 

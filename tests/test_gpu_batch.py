@@ -66,10 +66,11 @@ def test_seed_closed_orbits_are_fixed_points():
     1e-13 m) are fixed points of the oracle's one-turn map with that seed's
     errors bound as per-particle parameters."""
     from ufo_amd.batch_adjoint import SeedClosedOrbit
-    ring, prm, vals, counts, fam = lattices.dba_family_model(seeds=4)
+    ring, prm, vals, counts, fam = lattices.dba_family_model(seeds=16)
     co = SeedClosedOrbit(ring, ufo.KICK | ufo.DOUBLE_PRECISION, prm, vals, tol=1e-13)
     orbits = co.run()
     print("orbits", orbits, "steps", co.steps)
+    assert co.converged.all() and co.steps.max() <= 8
     pool = np.concatenate([orbits, vals], axis=1)
     t = runner.track(ring, pool, flags=ufo.KICK | ufo.DOUBLE_PRECISION | ufo.FIVED, turns=1,
                      parameters=["x", "px", "y", "py"] + prm, where=[-1])[:, 0, :4]

@@ -118,6 +118,7 @@ class SeedClosedOrbit(Simulation):
         self.seeds = self.seed_values.shape[0]
         self.table = batch.build_table(self.layout, self.seed_values)
         self.guess = np.zeros((self.seeds, 4))
+        self.iterations = int(iterations)
 
     def run(self, threads=1):
         d = self.device
@@ -128,6 +129,8 @@ class SeedClosedOrbit(Simulation):
         self.module().launch(self.kernel.name, (self.seeds + 63) // 64, 64, [tab, guess, orbits, steps, self.seeds])
         self.orbits = orbits.cpu().numpy()
         self.steps = steps.cpu().numpy()
+        # a seed that used every iteration did not reach the tolerance
+        self.converged = self.steps < self.iterations
         return self.orbits
 
 

@@ -46,8 +46,13 @@ def fodo_ring(cells=16, aperture=0.04, distinct=False):
     return Line("RING", body + [ap])
 
 
-DBA_QUADS = {"QF1": (0.25, 2.2), "QD1": (0.30, -1.0), "QF2": (0.50, 1.5),
-             "QF3": (0.50, 1.5), "QD2": (0.30, -1.0), "QF4": (0.25, 2.2)}
+# Working point: per-period tunes (0.4266, 0.3311), so Q = (1.707, 1.324) for
+# 4 periods and (6.826, 5.297) for 16, at least 0.17 from every integer and
+# half-integer in both rings (at QF2 = QF3 = 1.5 the 4-period vertical tune
+# sat at 0.972, where 50 um misalignments leave some seeds without a
+# closed orbit near the axis).
+DBA_QUADS = {"QF1": (0.25, 2.2), "QD1": (0.30, -1.0), "QF2": (0.50, 1.3),
+             "QF3": (0.50, 1.3), "QD2": (0.30, -1.0), "QF4": (0.25, 2.2)}
 DBA_SEXTS = {"SD1": -38.0, "SF1": 38.0, "SF2": 38.0, "SD2": -38.0}
 
 
