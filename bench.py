@@ -32,6 +32,10 @@ sys.path.insert(0, ROOT)
 PEAK_FP64_TFLOPS = 78.6
 
 
+def _progress(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -45,6 +49,7 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-seed-adjoint", action="store_true")
+    ap.add_argument("--no-fodo", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="MiB copied by one device-to-device copy before the timed region "
                          "(reference kernel for HBM traffic counters)")
@@ -141,15 +146,75 @@ def main():
         },
         "cpu_baseline": None,
     }
+    _progress(f"DA scan: {float(np.mean(kernel_ms)):.1f} ms per launch")
+    if not args.no_fodo:
+        out["fodo_track"] = fodo_track_timing(cpu=not args.no_cpu)
+        _progress("FODO Track done")
     if not args.no_adjoint:
         out["adjoint"] = adjoint_timing()
+        _progress("adjoint done")
     if not args.no_batched:
         out["batched"] = batched_timing()
+        _progress("batched seeds done")
     if not args.no_seed_adjoint:
         out["seed_adjoint"] = seed_adjoint_timing()
+        _progress("batched adjoint done")
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(ring, flags, pool, args.turns, E, args.cpu_seconds)
+        _progress("CPU baseline done")
     print(json.dumps(out))
+
+
+def fodo_track_timing(particles=4096, turns=1000, reps=3, cpu=True):
+    """The reference's CPU-runnable case: Track through the 16-cell FODO ring
+    (96 elements, KICK, 6D), 4096 particles (x, y ~ N(0, 1 mm), px, py ~
+    N(0, 0.1 mrad), dp ~ N(0, 1e-3), seed 0), where = [-1], 1000 turns.
+    GPU: kernel time (HIP events) and Track.run() wall time including the
+    196.6 MB copy of tracks back to the host; CPU: the oracle (gcc fast-math,
+    OpenMP) on the whole workload."""
+    import numpy as np
+    import torch
+    import ufo_amd as ufo
+    from ufo_amd import lattices
+    ring = lattices.fodo_ring(16)
+    flags = ufo.KICK | ufo.DOUBLE_PRECISION
+    coords = ["x", "px", "y", "py", "dp"]
+    bunch = lattices.gaussian_bunch(particles, seed=0)
+    tr = ufo.Track(ring, flags=flags, turns=turns, particles=particles, parameters=coords, where=[-1])
+    tr.parameters[:] = bunch
+    tr.run()
+    stream = torch.cuda.current_stream()
+    k_ms, run_ms = [], []
+    for _ in range(reps):
+        e = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+        torch.cuda.synchronize()
+        e[0].record(stream)
+        tr.launch()
+        e[1].record(stream)
+        torch.cuda.synchronize()
+        k_ms.append(e[0].elapsed_time(e[1]))
+        t = time.perf_counter()
+        tr.run()
+        run_ms.append(1e3 * (time.perf_counter() - t))
+    alive = np.isfinite(tr.tracks[:, -1, 0])
+    nan_turn = np.argmax(~np.isfinite(tr.tracks[:, :, 0]), axis=1)
+    pt = int(np.where(alive, turns, nan_turn + 1).sum())
+    E = tr.kernel.elements
+    out = {"workload": f"16-cell FODO ({E} elements, KICK, 6D), {particles} particles, {turns} turns, where=[-1]",
+           "gpu_kernel_ms": round(float(np.median(k_ms)), 3), "gpu_run_ms": round(float(np.median(run_ms)), 3),
+           "gpu_pet_per_s": pt * E / (float(np.median(k_ms)) * 1e-3), "survivors": int(alive.sum())}
+    if cpu:
+        from oracle import runner
+        threads = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
+        runner.track(ring, bunch[:64], flags=flags, turns=turns, parameters=coords, where=[-1], mode="fast")
+        t = time.perf_counter()
+        runner.track(ring, bunch, flags=flags, turns=turns, parameters=coords, where=[-1], mode="fast")
+        dt = time.perf_counter() - t
+        out["cpu_ms"] = round(1e3 * dt, 1)
+        out["cpu_pet_per_s"] = pt * E / dt
+        out["cpu_threads"] = threads
+        out["cpu_kind"] = "port (oracle, whole workload)"
+    return out
 
 
 def batched_timing(seeds=256, turns=2048, reps=2):
@@ -327,6 +392,17 @@ def measured_traffic(args):
     return None
 
 
+def _host_cpu():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(ring, flags, pool, turns, E, budget_s):
     """The oracle (restated reference kernels, gcc fast-math build, OpenMP)
     on an evenly strided sample of the grid, sized to ~budget_s seconds."""
@@ -348,6 +424,7 @@ def cpu_baseline(ring, flags, pool, turns, E, budget_s):
     dt = time.perf_counter() - t
     pt = int(np.minimum(lost.astype(np.int64) + 1, turns).sum())
     return {"value": pt * E / dt, "unit": "particle-element-turns/s", "cores": threads, "kind": "port",
+            "host_cpu": _host_cpu(),
             "sample": f"{len(sample)} grid points (every {stride}th of the {len(pool)}-point grid), "
                       f"{turns} turns, {dt:.1f} s"}
 

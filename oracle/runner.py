@@ -119,7 +119,9 @@ class Optics:
     def __init__(self, line, where=(), periodic=True, parameters=(),
                  flags=refgen.FLAG_LINEAR | refgen.FLAG_ACHROMATIC, mode="strict"):
         self.periodic = periodic
-        src, self.count, self.where = refgen.optics_source(line, where, periodic, parameters, flags)
+        src, self.count, where = refgen.optics_source(line, where, periodic, parameters, flags)
+        flat = line.flatten()
+        self.where = [(idx, flat[int(idx)]) for idx in where]    # ufo/optics.py:82
         self.lib = compile_c(src, mode, _fp32(flags))
         self.dtype = _dtype(flags)
         self.parameters = np.zeros((1, max(1, len(parameters))), dtype=self.dtype)

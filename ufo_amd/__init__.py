@@ -10,6 +10,9 @@ from .elements import (Element, Marker, Drift, Multipole, Quadrupole, Sbend, Rbe
                        Octupole, Wire, Aperture, CodeElement)
 from .lattice import Lattice, Line, Beam, dump
 from .mad import load as load_mad
+from .radiation import emittance, RadiationIntegrals
+from .chroma import chromaticity
+from .rdt import rdt
 
 DEFAULT_QUADRUPOLE_SLICES = 8
 DEFAULT_BEND_SLICES = 8

@@ -34,7 +34,7 @@ import numpy as np
 import torch
 
 from . import codegen, ir
-from .codegen import Env, _fma, _negs, is_zero, emit_op, kick_coeffs
+from .codegen import Env, _fma, _fma_neg, _negs, is_zero, emit_op, kick_coeffs
 from .flags import DOUBLE_PRECISION
 from .sim import Simulation
 
@@ -189,10 +189,10 @@ def emit_kick_reverse(op: ir.Kick, env: Env, out: List[str], g: _Grad):
                 px, py = "px", "py"
                 if wr is not None:
                     px = _fma(wr, "x", px)
-                    py = _fma(_negs(wr), "y", py)
+                    py = _fma_neg(wr, "y", py)
                 if wi is not None:
-                    px = _fma(_negs(wi), "y", px)
-                    py = _fma(_negs(wi), "x", py)
+                    px = _fma_neg(wi, "y", px)
+                    py = _fma_neg(wi, "x", py)
                 if c0r is not None:
                     px = f"({px} + {c0r})"
                 if c0i is not None:
@@ -232,7 +232,7 @@ def emit_kick_reverse(op: ir.Kick, env: Env, out: List[str], g: _Grad):
     if top >= 1:
         lx, ly = "lx", "ly"
         if dr is not None:
-            lx = _fma(_negs(dr), "lpx", lx)
+            lx = _fma_neg(dr, "lpx", lx)
             ly = _fma(dr, "lpy", ly)
         if di is not None:
             lx = _fma(di, "lpy", lx)
@@ -272,7 +272,7 @@ def emit_kick_reverse_factored(op: ir.Kick, env: Env, out: List[str], g: _Grad):
     dr, di = codegen.kick_derivative(op, env, lines)
     lx, ly = "lx", "ly"
     if dr is not None:
-        lx = _fma(_negs(dr), "slx", lx)
+        lx = _fma_neg(dr, "slx", lx)
         ly = _fma(dr, "sly", ly)
     if di is not None:
         lx = _fma(di, "sly", lx)
@@ -340,13 +340,16 @@ def _reverse_layout(ops, env: Env, ntheta: int):
     out: List[str] = []
     if reps >= 2:
         out += segment(ops[head + p * reps:])
-        out.append("#pragma clang loop unroll(disable)")
-        out.append(f"for (int rep = {reps - 1}; rep >= 0; --rep) {{")
         env.stride = ("rep", stride)
         try:
-            out += ["    " + s for s in segment(ops[head:head + p], "rep")]
+            inner = segment(ops[head:head + p], "rep")
         finally:
             env.stride = None
+        pre, top, end = codegen.prefetch_lines(env, str(reps - 1), "(rep > 0 ? rep - 1 : 0)")
+        out += pre
+        out.append("#pragma clang loop unroll(disable)")
+        out.append(f"for (int rep = {reps - 1}; rep >= 0; --rep) {{")
+        out += ["    " + s for s in top + inner + end]
         out.append("}")
         out += segment(ops[:head])
     else:
@@ -354,11 +357,55 @@ def _reverse_layout(ops, env: Env, ntheta: int):
     return out, (head, p, reps, stride)
 
 
-def adjoint_kernels(line, flags, turns, coords, lattice_parameters, dp=0.0, every=32, block=64):
-    """(forward kernel, reverse kernel, number of checkpoints) of a loss on
-    the final state.  ``coords`` are the per-particle coordinate columns
-    (names among x, px, y, py, dp), ``lattice_parameters`` the shared
-    (label-or-index, name) parameters, in the order of theta."""
+LDS_TABLE_MAX = 4096   # coefficient-table slots staged in LDS (32 KiB); larger tables stay in HBM
+
+
+def _table_kernel(env: Env, flat, period, reps, stride, F, S, name="ufo_adj_table"):
+    """Kernel evaluating every coefficient of the table from theta, in the
+    arithmetic order the tracking kernels would use inline: lane r < reps
+    writes repetition r's period slots, lane 0 also the flat ones."""
+    p = Env(fp32=False, consts=dict(env.consts), shared_offset=env.shared_offset)
+    lines = []
+    if flat:
+        lines.append("if (r == 0) {")
+        lines += [f"    ct[{k}] = {p.raw(e)};" for k, e in enumerate(flat)]
+        lines.append("}")
+    if period and reps:
+        p.stride = ("r", stride)
+        lines.append(f"if (r < {reps}) {{")
+        lines += [f"    ct[{F} + {j} + r * {S}] = {p.raw(e)};" for j, e in enumerate(period)]
+        lines.append("}")
+    return f"""
+extern "C" __global__ void __launch_bounds__(64)
+{name}(const ufloat *__restrict__ th, ufloat *__restrict__ ct)
+{{
+    const int r = blockIdx.x * 64 + threadIdx.x;
+{chr(10).join('    ' + x for x in lines)}
+}}
+"""
+
+
+def _stage(ns: int, block: int) -> str:
+    """Coefficient table into LDS (cs), or cs aliasing the table in HBM."""
+    if ns <= LDS_TABLE_MAX:
+        return (f"__shared__ ufloat cs[{ns}];\n"
+                f"    for (int i = threadIdx.x; i < {ns}; i += {block}) cs[i] = ct[i];\n"
+                f"    __syncthreads();")
+    return "const ufloat *__restrict__ cs = ct;"
+
+
+def adjoint_kernels(line, flags, turns, coords, lattice_parameters, dp=0.0, every=32, block=256):
+    """(forward kernel, reverse kernel, table kernel, table layout, number of
+    checkpoints) of a loss on the final state.  ``coords`` are the
+    per-particle coordinate columns (names among x, px, y, py, dp),
+    ``lattice_parameters`` the shared (label-or-index, name) parameters, in
+    the order of theta.
+
+    Every coefficient that depends on theta alone is evaluated once per call
+    by the table kernel (``ufo_adj_table``) into a table ct[], which both
+    tracking kernels stage in LDS; inside the period loop the coefficients of
+    the next repetition are loaded one iteration ahead into registers, so no
+    wait on a load sits in the loop."""
     if not flags & DOUBLE_PRECISION:
         raise ValueError("the adjoint tracker computes in FP64 (DOUBLE_PRECISION)")
     coords = list(coords)
@@ -370,28 +417,46 @@ def adjoint_kernels(line, flags, turns, coords, lattice_parameters, dp=0.0, ever
             raise NotImplementedError("raw user code has no adjoint")
     nck = (turns + every - 1) // every + 1
     src = codegen._coord_sources(coords, flags, dp)
+    table: list = []
+    slots: dict = {}
 
     def make_env():
         env = codegen._env_for(flags, src, None)
         env.shared_offset = len(coords)
+        env.table, env.period_slots = table, slots      # one table for both kernels
+        env.table_name = "cs"
+        env.prefetch = True
+        env.factor_chroma = False
         return env
 
     P = max(1, len(coords))
     nth = max(1, len(shared))
     pre = codegen._PRELUDE.format(ufloat="double", nan='__builtin_nan("")')
 
-    # forward
-    env = make_env()
-    fbody, period = codegen._layout(ops, env)
-    load = codegen._load_block(env, src, P, "row", "    ")
+    envf = make_env()
+    fbody, period = codegen._layout(ops, envf)
+    envb = make_env()
+    rbody, rperiod = _reverse_layout(ops, envb, nth)
+    head, p, reps, stride = rperiod
+    F, S = len(table), len(slots)
+    reps = int(reps) if reps >= 2 and S else 0
+    ns = max(1, F + reps * S)
+    layout = {"flat": [e for e, _ in table], "period": sorted(slots, key=slots.get), "reps": reps,
+              "stride": int(stride), "F": F, "S": S, "nslots": ns}
+
+    def fill(text):
+        return text.replace("@PB@", str(F)).replace("@PS@", str(max(S, 1)))
+
+    load = codegen._load_block(envf, src, P, "row", "    ")
     store_ck = ("{ ufloat *c = ckpt + (size_t)ck * 4 * np + idx; c[0] = x; c[np] = px; c[2 * np] = y; "
                 "c[3 * np] = py; }")
-    fwd = pre + f"""
+    fwd = pre + fill(f"""
 extern "C" __global__ void __launch_bounds__(256)
-ufo_adj_fwd(const ufloat *__restrict__ pool, const ufloat *__restrict__ th, ufloat *__restrict__ ckpt,
+ufo_adj_fwd(const ufloat *__restrict__ pool, const ufloat *__restrict__ ct, ufloat *__restrict__ ckpt,
             ufloat *__restrict__ xout, int *__restrict__ lost, int particles)
 {{
-    {codegen._decls(env)}
+    {_stage(ns, 256)}
+    {codegen._decls(envf)}
     const int idx = blockIdx.x * 256 + threadIdx.x;
     if (idx >= particles) return;
     const size_t np = (size_t)particles;
@@ -409,28 +474,25 @@ ufo_adj_fwd(const ufloat *__restrict__ pool, const ufloat *__restrict__ th, uflo
     o[0] = x; o[1] = px; o[2] = y; o[3] = py;
     lost[idx] = turn;
 }}
-"""
+""")
     fk = codegen.Kernel(name="ufo_adj_fwd", src=fwd, kind="adj_fwd", fp32=False,
-                        flops_per_turn=codegen._count(ops, env), elements=codegen.element_count(line), block=256,
+                        flops_per_turn=codegen._count(ops, envf), elements=codegen.element_count(line), block=256,
                         period=period)
 
-    # reverse
-    env = make_env()
-    rbody, rperiod = _reverse_layout(ops, env, nth)
-    load = codegen._load_block(env, src, P, "row", "        ")
-    bwd = pre + f"""
+    load = codegen._load_block(envb, src, P, "row", "        ")
+    bwd = pre + fill(f"""
 extern "C" __global__ void __launch_bounds__({block})
-ufo_adj_bwd(const ufloat *__restrict__ pool, const ufloat *__restrict__ th, const ufloat *__restrict__ ckpt,
+ufo_adj_bwd(const ufloat *__restrict__ pool, const ufloat *__restrict__ ct, const ufloat *__restrict__ ckpt,
             const ufloat *__restrict__ lam, const int *__restrict__ lost, ufloat *__restrict__ gx0,
             ufloat *__restrict__ gpart, int particles)
 {{
     __shared__ ufloat acc[{nth} * {ACCW}];
     for (int i = threadIdx.x; i < {nth} * {ACCW}; i += {block}) acc[i] = 0.0;
-    __syncthreads();
+    {_stage(ns, block)}
     const int idx = blockIdx.x * {block} + threadIdx.x;
     const int slot = threadIdx.x % {ACCW};
     if (idx < particles && lost[idx] == {turns}) {{
-        {codegen._decls(env)}
+        {codegen._decls(envb)}
         const size_t np = (size_t)particles;
         const ufloat *__restrict__ row = pool + (size_t)idx * {P};
 {chr(10).join(load)}
@@ -455,11 +517,14 @@ ufo_adj_bwd(const ufloat *__restrict__ pool, const ufloat *__restrict__ th, cons
         gpart[(size_t)blockIdx.x * {nth} + j] = s;
     }}
 }}
-"""
+""")
     bk = codegen.Kernel(name="ufo_adj_bwd", src=bwd, kind="adj_bwd", fp32=False,
-                        flops_per_turn=codegen._count(ops, env), elements=codegen.element_count(line), block=block,
+                        flops_per_turn=codegen._count(ops, envb), elements=codegen.element_count(line), block=block,
                         period=rperiod)
-    return fk, bk, nck
+    tsrc = pre + _table_kernel(envf, layout["flat"], layout["period"], reps, stride, F, max(S, 1))
+    tk = codegen.Kernel(name="ufo_adj_table", src=tsrc, kind="adj_table", fp32=False, flops_per_turn=0,
+                        elements=0, block=64)
+    return fk, bk, tk, layout, nck
 
 
 # ---------------------------------------------------------------------------
@@ -488,16 +553,27 @@ class AdjointTracker(Simulation):
         self.coords = list(coords)
         self.lattice_parameters = list(lattice_parameters)
         self.every = int(every)
-        self.kfwd, self.kbwd, self.ncheck = adjoint_kernels(line, flags, self.turns, self.coords,
-                                                            self.lattice_parameters, dp, self.every)
+        self.kfwd, self.kbwd, self.ktab, self.layout, self.ncheck = adjoint_kernels(
+            line, flags, self.turns, self.coords, self.lattice_parameters, dp, self.every)
         self.kernel = self.kfwd
         self._mods = None
         self._saved = None
 
     def modules(self):
         if self._mods is None:
-            self._mods = (self.ctx.module(self.kfwd.src, self.options), self.ctx.module(self.kbwd.src, self.options))
+            self._mods = (self.ctx.module(self.kfwd.src, self.options), self.ctx.module(self.kbwd.src, self.options),
+                          self.ctx.module(self.ktab.src, self.options))
         return self._mods
+
+    def coefficients(self, theta):
+        """The coefficient table of theta (device tensor [nslots]), computed
+        by the table kernel on the current stream."""
+        th = self._theta(theta)
+        ct = torch.zeros(self.layout["nslots"], dtype=torch.float64, device=self.device)
+        if self.layout["F"] or self.layout["reps"]:
+            grid = max(1, (max(self.layout["reps"], 1) + 63) // 64)
+            self.modules()[2].launch(self.ktab.name, grid, 64, [th, ct])
+        return ct
 
     def _theta(self, theta):
         th = torch.as_tensor(theta, dtype=torch.float64, device=self.device).contiguous()
@@ -506,21 +582,21 @@ class AdjointTracker(Simulation):
         return th
 
     def forward(self, x0, theta):
-        fwd, _ = self.modules()
+        fwd = self.modules()[0]
         x0 = torch.as_tensor(x0, dtype=torch.float64, device=self.device).contiguous()
         n = x0.shape[0]
         pool = x0 if x0.shape[1] > 0 else torch.zeros((n, 1), dtype=torch.float64, device=self.device)
-        th = self._theta(theta)
+        ct = self.coefficients(theta)
         ckpt = torch.empty((self.ncheck, 4, n), dtype=torch.float64, device=self.device)
         xf = torch.empty((n, 4), dtype=torch.float64, device=self.device)
         lost = torch.empty((n,), dtype=torch.int32, device=self.device)
         if n:
-            fwd.launch(self.kfwd.name, (n + 255) // 256, 256, [pool, th, ckpt, xf, lost, n])
-        self._saved = (pool, th, ckpt, lost)
+            fwd.launch(self.kfwd.name, (n + 255) // 256, 256, [pool, ct, ckpt, xf, lost, n])
+        self._saved = (pool, ct, ckpt, lost)
         return xf, lost
 
     def backward(self, lam):
-        _, bwd = self.modules()
+        bwd = self.modules()[1]
         pool, th, ckpt, lost = self._saved
         n = pool.shape[0]
         lam = torch.nan_to_num(torch.as_tensor(lam, dtype=torch.float64, device=self.device), nan=0.0).contiguous()

@@ -77,6 +77,10 @@ class Env:
     diff_cols: Optional[list] = None  # parameter columns with gradients (None: every shared one)
     no_loop: bool = False            # emit the turn flat (no period loop)
     stride: Optional[tuple] = None   # (loop variable, parameter stride) inside a strided period loop
+    table_name: str = "ct"           # array holding the coefficient table
+    prefetch: bool = False           # period-loop coefficients in registers K<j>, loaded one repetition ahead
+    factor_chroma: bool = True       # table mode: apply 1/(1+dp) to whole kick sums
+    products: Dict[tuple, str] = field(default_factory=dict)  # prefetch mode: (slot, variable) -> register
 
     def f(self, e) -> Expr:
         return fold(e, self.fp32, self.consts)
@@ -128,7 +132,12 @@ class Env:
             return self.raw(e)
         if e.has_raw() or e.uses_var("turn"):
             return self.raw(e)
-        if e.params() and (self.param_mem or (self.stride is not None and self.stride[1] != 0)):
+        in_loop = self.stride is not None and self.stride[1] != 0
+        if self.table is not None and in_loop and e.params() and min(e.params()) >= self.shared_offset:
+            # a table coefficient combined with per-particle variables: built
+            # from the parts, evaluated where it is used
+            return self._compose(e)
+        if e.params() and (self.param_mem or in_loop):
             return self.raw(e)
         if k == "neg":
             return f"(-{self.val(e.t[1])})"
@@ -146,19 +155,55 @@ class Env:
             self.inv_code.append(f"{name} = {text};")
         return name
 
+    def _compose(self, e) -> str:
+        k = e.t[0]
+        # (X * Y) / 2^n and (X * Y) * 2^n, X a table coefficient, Y
+        # per-particle: the power-of-two scaling moves onto X (exact), so the
+        # table holds it and one multiply remains
+        if k in ("/", "*") and e.t[2].is_const() and _pow2(e.t[2].value) and e.t[1].t[0] == "*":
+            a, b = e.t[1].t[1], e.t[1].t[2]
+            for X, Y in ((a, b), (b, a)):
+                if X.params() and not (X.uses_var("oodppo") or X.uses_var("dp")) and not X.has_raw() \
+                        and min(X.params()) >= self.shared_offset:
+                    return self._compose(ir.Expr(("*", ir.Expr((k, X, e.t[2])), Y)))
+        if k == "*" and self.prefetch:
+            # table coefficient times a kernel variable: the product itself
+            # is the loop-carried register, formed when the coefficient
+            # arrives, off the critical path of the next iteration
+            a, b = e.t[1], e.t[2]
+            for X, Y in ((a, b), (b, a)):
+                if Y.t[0] == "v" and X.params() and not X.has_raw() and not (X.uses_var("oodppo") or X.uses_var("dp")) \
+                        and min(X.params()) >= self.shared_offset:
+                    kreg = self.table_slot(X)
+                    if kreg.startswith("K"):
+                        key = (int(kreg[1:]), Y.t[1])
+                        if key not in self.products:
+                            self.products[key] = f"KP{len(self.products)}"
+                        return self.products[key]
+        if k == "neg":
+            return f"(-{self.val(e.t[1])})"
+        if k == "f":
+            return f"{e.t[1]}({self.val(e.t[2])})"
+        if k == "?":
+            return f"({self.val(e.t[1])} ? {self.val(e.t[2])} : {self.val(e.t[3])})"
+        return f"({self.val(e.t[1])} {k} {self.val(e.t[2])})"
+
     def table_slot(self, e) -> str:
-        """Slot of a per-seed coefficient in the table ct[]; inside a strided
-        period loop the slot repeats per repetition."""
+        """Slot of a coefficient in the table; inside a strided period loop
+        the slot repeats per repetition (register K<j> under prefetch)."""
+        t = self.table_name
         if self.stride is not None and self.stride[1] != 0:
             slots = self.period_slots
             if e not in slots:
                 slots[e] = len(slots)
-            return f"ct[@PB@ + {slots[e]} + {self.stride[0]} * @PS@]"
+            if self.prefetch:
+                return f"K{slots[e]}"
+            return f"{t}[@PB@ + {slots[e]} + {self.stride[0]} * @PS@]"
         for k, (ex, _) in enumerate(self.table):
             if ex == e:
-                return f"ct[{k}]"
+                return f"{t}[{k}]"
         self.table.append((e, None))
-        return f"ct[{len(self.table) - 1}]"
+        return f"{t}[{len(self.table) - 1}]"
 
     def neg(self, e) -> str:
         e = self.f(e)
@@ -259,7 +304,7 @@ def chroma_factored(op, env: Env) -> bool:
     """In coefficient-table mode a per-particle 1/(1+dp) cannot be folded
     into the per-seed coefficients: the kick is computed without it and
     scaled as a whole (every coefficient carries the same factor)."""
-    return (env.table is not None and isinstance(op, ir.Kick) and op.chroma
+    return (env.table is not None and env.factor_chroma and isinstance(op, ir.Kick) and op.chroma
             and not env.f(ir.OODPPO).is_const())
 
 
@@ -301,7 +346,7 @@ def emit_kick(op: ir.Kick, env: Env, out: List[str]):
     # px -= wr*x - wi*y + d0r ; py += wr*y + wi*x + d0i
     px = "px"
     if wr is not None:
-        px = _fma(_negs(wr), "x", px)
+        px = _fma_neg(wr, "x", px)
     if wi is not None:
         px = _fma(wi, "y", px)
     if d0r is not None:
@@ -383,14 +428,30 @@ def _negs(s: str) -> str:
     return s[1:] if s.startswith("-") else f"-{s}"
 
 
+def _pow2(v: float) -> bool:
+    m, _ = math.frexp(abs(v))
+    return v != 0.0 and m == 0.5
+
+
+def _fma_neg(coef: str, var: str, acc: str) -> str:
+    """acc - coef * var with the sign on the variable operand (a free source
+    modifier; a negated product would cost the compiler a second multiply).
+    fma(c, -v, a) and fma(-c, v, a) round identically."""
+    if coef.startswith("-"):
+        return _fma(coef[1:], var, acc)
+    return _fma(coef, f"-{var}", acc)
+
+
 def _chain(terms, add):
     """a1*b1 (+/-) a2*b2 + add as nested fma; None when every term is zero."""
     acc = add
     for coef, var, negate in reversed(terms):
         if coef is None:
             continue
-        c = _negs(coef) if negate else coef
-        acc = _fma(c, var, acc) if acc is not None else f"({c} * {var})"
+        if acc is None:
+            acc = f"({_negs(coef) if negate else coef} * {var})"
+        else:
+            acc = _fma_neg(coef, var, acc) if negate else _fma(coef, var, acc)
     return acc
 
 
@@ -610,6 +671,14 @@ typedef {ufloat} ufloat;
 #ifndef NAN
 #define NAN UFO_NAN
 #endif
+// a value the optimiser may not trace back to its definition (no
+// instruction is emitted): keeps a prefetched, loop-carried coefficient from
+// being folded back into a load at its use
+__device__ static inline double ufo_opaque(double v)
+{{
+    __asm__("" : "+v"(v));
+    return v;
+}}
 """
 
 
@@ -638,8 +707,6 @@ def _layout(body_ops, env, observe=None, loop_var="rep", emit=None):
     if reps >= 2:
         for op in body_ops[:head]:
             emit(op, env, out, observe)
-        out.append("#pragma clang loop unroll(disable)")
-        out.append(f"for (int {loop_var} = 0; {loop_var} < {reps}; ++{loop_var}) {{")
         inner: List[str] = []
         env.stride = (loop_var, stride)
         try:
@@ -647,7 +714,12 @@ def _layout(body_ops, env, observe=None, loop_var="rep", emit=None):
                 emit(op, env, inner, observe)
         finally:
             env.stride = None
-        out += ["    " + s for s in inner]
+        nxt = f"({loop_var} + 1 < {reps} ? {loop_var} + 1 : {loop_var})"
+        pre, top, end = prefetch_lines(env, "0", nxt)
+        out += pre
+        out.append("#pragma clang loop unroll(disable)")
+        out.append(f"for (int {loop_var} = 0; {loop_var} < {reps}; ++{loop_var}) {{")
+        out += ["    " + s for s in top + inner + end]
         out.append("}")
         for op in body_ops[head + p * reps:]:
             emit(op, env, out, observe)
@@ -655,6 +727,27 @@ def _layout(body_ops, env, observe=None, loop_var="rep", emit=None):
         for op in body_ops:
             emit(op, env, out, observe)
     return out, (head, p, reps)
+
+
+def prefetch_lines(env: Env, first: str, nxt: str):
+    """Register rotation of the period-loop table coefficients (prefetch
+    mode): K<j> holds repetition ``first``'s value before the loop; each
+    iteration loads the next repetition's N<j> at its top (so the load is in
+    flight while the body runs) and moves it into K<j> at its end.  Returns
+    (before the loop, loop top, loop end) lines; empty without prefetch."""
+    if not env.prefetch or not env.period_slots:
+        return [], [], []
+    t = env.table_name
+    js = sorted(env.period_slots.values())
+    # the rotation passes through ufo_opaque: otherwise the compiler folds
+    # the loop-carried value back into a load at the top of the iteration
+    pre = [f"ufloat K{j} = {t}[@PB@ + {j} + ({first}) * @PS@];" for j in js]
+    top = [f"const int nrep_ = {nxt};"] + [f"const ufloat N{j} = {t}[@PB@ + {j} + nrep_ * @PS@];" for j in js]
+    end = [f"K{j} = ufo_opaque(N{j});" for j in js]
+    for (j, var), name in sorted(env.products.items(), key=lambda kv: kv[1]):
+        pre.append(f"ufloat {name} = K{j} * {var};")
+        end.append(f"{name} = K{j} * {var};")
+    return pre, top, end
 
 
 def _coord_sources(parameters, flags, dp):

@@ -24,7 +24,7 @@ import math
 from typing import List
 
 from . import codegen, ir
-from .codegen import Env, _fma, _negs, is_zero, emit_op, kick_coeffs
+from .codegen import Env, _fma, _fma_neg, _negs, is_zero, emit_op, kick_coeffs
 
 def _loop(body: str) -> str:
     return f"_Pragma(\"unroll\") for (int k = 0; k < NT; ++k) {{ {body} }}"
@@ -112,7 +112,7 @@ def emit_kick_tangent(op: ir.Kick, env: Env, out: List[str]):
     px = "tpx[k]"
     py = "tpy[k]"
     if wr is not None:
-        px = _fma(_negs(wr), "tx[k]", px)
+        px = _fma_neg(wr, "tx[k]", px)
         py = _fma(wr, "ty[k]", py)
     if wi is not None:
         px = _fma(wi, "ty[k]", px)
