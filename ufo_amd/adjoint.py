@@ -431,7 +431,7 @@ def adjoint_kernels(line, flags, turns, coords, lattice_parameters, dp=0.0, ever
 
     P = max(1, len(coords))
     nth = max(1, len(shared))
-    pre = codegen._PRELUDE.format(ufloat="double", nan='__builtin_nan("")')
+    pre = codegen._PRELUDE.format(ufloat="double", nan='__builtin_nan("")') + codegen.OPAQUE
 
     envf = make_env()
     fbody, period = codegen._layout(ops, envf)

@@ -671,14 +671,16 @@ typedef {ufloat} ufloat;
 #ifndef NAN
 #define NAN UFO_NAN
 #endif
-// a value the optimiser may not trace back to its definition (no
-// instruction is emitted): keeps a prefetched, loop-carried coefficient from
-// being folded back into a load at its use
-__device__ static inline double ufo_opaque(double v)
-{{
+"""
+
+# Prefetch-mode kernels only: a value the optimiser may not trace back to its
+# definition (no instruction is emitted), which keeps a prefetched,
+# loop-carried coefficient from being folded back into a load at its use.
+OPAQUE = """__device__ static inline double ufo_opaque(double v)
+{
     __asm__("" : "+v"(v));
     return v;
-}}
+}
 """
 
 
