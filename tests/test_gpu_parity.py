@@ -217,6 +217,54 @@ def test_optics_consumers_alba():
         np.testing.assert_allclose(np.asarray(a, dtype=complex), np.asarray(b, dtype=complex), rtol=1e-9)
 
 
+@pytest.mark.parametrize("flags", [ufo.DOUBLE_PRECISION, ufo.KICK | ufo.DOUBLE_PRECISION,
+                                   ufo.KICK | ufo.DOUBLE_PRECISION | ufo.EXACT])
+def test_elements_lattice(flags):
+    """tests/elements.py's lattice (quadrupole, sbend with pole faces, drift,
+    marker, sextupole, octupole multipole) tracked 5 passes with an
+    observation point on the marker (never written, as in the reference)."""
+    lat = ufo.Lattice(os.path.join(OPTICS_DIR, "elements.mad"))
+    rng = np.random.default_rng(11)
+    pool = np.stack([rng.normal(0, 1e-3, 64), rng.normal(0, 1e-4, 64), rng.normal(0, 1e-3, 64),
+                     rng.normal(0, 1e-4, 64), rng.normal(0, 1e-3, 64)], 1)
+    gpu, strict, fast, moved = _track_both(lat.TEST, pool, flags, 5, COORD6, [1, 3.5, -1])
+    r = compare_tracks(gpu, strict, fast, moved)
+    print(f"flags {flags}: " + summary(r))
+    assert_parity(r)
+    assert np.all(np.isnan(gpu[:, 10:, :]))          # samples 10..14 are never written
+
+
+@pytest.mark.parametrize("double", [False, True])
+def test_octupole_ring_closed_form(double):
+    """tests/oct.py: oct.mad, FIVED, x0 = 0.01, 10 turns, against the exact
+    map (mpmath, 200 bits)."""
+    from test_reference_cases import _oct_closed_form
+    lat = ufo.Lattice(os.path.join(OPTICS_DIR, "oct.mad"))
+    flags = ufo.FIVED | (ufo.DOUBLE_PRECISION if double else 0)
+    tr = ufo.Track(lat.RING, turns=10, particles=2, where=[-1], flags=flags, parameters=["x"])
+    tr.parameters[:, 0] = 0.01
+    tr.run(threads=1)
+    exact = _oct_closed_form(0.01, 10, 10765.5878, 0.2876, 200)
+    tol = 1e-13 if double else 2e-5
+    np.testing.assert_allclose(tr.tracks[0][:, 0], exact[:, 0], rtol=tol, atol=tol * 0.01)
+    np.testing.assert_allclose(tr.tracks[0][:, 1], exact[:, 1], rtol=tol, atol=tol * 1e-4)
+
+
+def test_optics_periodic_equals_propagated():
+    """tests/optics2.py on the HIP Optics: the propagated solution started
+    from the periodic end values returns them."""
+    alba = ufo.Lattice(os.path.join(OPTICS_DIR, "alba.mad"))
+    fl = ufo.LINEAR | ufo.ACHROMATIC | ufo.DOUBLE_PRECISION
+    per = ufo.Optics(alba.RING, where=[-1], periodic=True, flags=fl)
+    per.run(threads=1)
+    pro = ufo.Optics(alba.RING, where=[-1], periodic=False, flags=fl)
+    for a in ("ax", "ay", "bx", "by", "dx", "dpx"):
+        setattr(pro, a + "0", getattr(per, a)[-1])
+    pro.run(threads=1)
+    for a in ("ax", "ay", "bx", "by", "dx", "dpx"):
+        assert abs(getattr(per, a)[-1] - getattr(pro, a)[-1]) < 1e-9 * max(1.0, abs(getattr(per, a)[-1]))
+
+
 def test_sa_threads_and_refill():
     """Results do not depend on the number of resident lanes (persistent
     refill) — run with 64 lanes and with the full device."""
