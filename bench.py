@@ -250,7 +250,8 @@ def batched_timing(seeds=256, turns=2048, reps=2):
             "tflops": pt * ba.kernel.flops_per_turn / dt / 1e12,
             "flop_per_particle_turn": ba.kernel.flops_per_turn, "executed_particle_turns": pt,
             "survivors": int((lost == turns).sum()), "block_turns": ba.block_turns,
-            "waves_per_simd": ba.waves_per_simd}
+            "waves_per_simd": ba.waves_per_simd, "particles_per_lane": ba.particles_per_lane,
+            "workgroup": ba.workgroup}
 
 
 def seed_adjoint_timing(seeds=64, per_seed=8192, turns=1024, every=32):

@@ -39,7 +39,7 @@ def test_batched_matches_oracle_per_particle_parameters():
     assert np.array_equal(lost[regular], strict[regular])
 
 
-def test_graph_replay_and_block_size_do_not_change_results():
+def test_graph_replay_and_launch_shape_do_not_change_results():
     turns = 256
     _, _, _, counts, _, ba, pool = _setup(5, (64, 700), turns)
     ba.run()
@@ -48,6 +48,12 @@ def test_graph_replay_and_block_size_do_not_change_results():
     assert np.array_equal(ba.lost, first)
     ba.block_turns = 32
     ba.run()                                # another block size: same physics
+    assert np.array_equal(ba.lost, first)
+    ba.particles_per_lane, ba.workgroup = 2, 64
+    ba.run()                                # two particles per lane, 64-lane workgroups
+    assert np.array_equal(ba.lost, first)
+    ba.particles_per_lane, ba.workgroup = 2, 256
+    ba.run()
     assert np.array_equal(ba.lost, first)
     assert np.sum(first < turns) > 0
 
@@ -58,6 +64,7 @@ def test_autotuner_picks_a_candidate():
     ba.block_turns = None
     ba.run()
     assert ba.block_turns in (32, 64, 128) and ba.waves_per_simd in (4, 8)
+    assert ba.particles_per_lane in (1, 2) and ba.workgroup in (64, 256)
     assert np.all(ba.lost <= turns)
 
 

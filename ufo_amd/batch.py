@@ -29,6 +29,7 @@ from __future__ import annotations
 import hashlib
 import json
 import os
+import re
 import time
 
 import numpy as np
@@ -39,15 +40,36 @@ from .codegen import Env
 from .flags import DOUBLE_PRECISION, FIVED
 from .sim import Simulation, host_buffer
 
-TILE = 64
+TILE = 64          # particles per tile and lane-particle (a tile is one wave's work)
+PAIR_RE = re.compile(r"\b(x|px|y|py|t\d+)\b")
 
 
 def _turn_ops(line, flags, parameters):
     return [op for piece in line.method(flags=flags, parameters=parameters) for op in piece]
 
 
+def _paired(lines):
+    """The turn body for two particles per lane: every statement is emitted
+    for particle A and again, right after it, for particle B (coordinates
+    x, px, y, py and temporaries renamed), so the two independent streams
+    interleave; loop headers, braces and pragmas are shared."""
+    out = []
+    for ln in lines:
+        st = ln.strip()
+        if st.startswith("#pragma") or st.endswith("{") or st == "}":
+            out.append(ln)
+        else:
+            out.append(ln)
+            out.append(PAIR_RE.sub(lambda m: m.group(1) + "b" if not m.group(1).startswith("t")
+                                   else "tb" + m.group(1)[1:], ln))
+    return out
+
+
 def batch_kernels(line, flags, coords, lattice_parameters, dp=0.0, block=256):
-    """Kernel source, table layout and facts of the batched tracker."""
+    """Kernel source, table layout and facts of the batched tracker: the
+    compaction and scan kernels and two turn kernels, ``ufo_bt_turns`` (one
+    particle per lane) and ``ufo_bt_turns2`` (two particles per lane, tiles of
+    128), launched with workgroups of up to ``block`` lanes."""
     coords = list(coords)
     parameters = coords + list(lattice_parameters)
     ops = _turn_ops(line, flags, parameters)
@@ -90,7 +112,7 @@ def batch_kernels(line, flags, coords, lattice_parameters, dp=0.0, block=256):
 // one workgroup per seed: compact the seed's alive particles, count tiles
 extern "C" __global__ void __launch_bounds__(256)
 ufo_bt_compact(const int *__restrict__ lost, const int *__restrict__ seed_off, int *__restrict__ idx,
-               int *__restrict__ seed_cnt, int *__restrict__ seed_tiles, int turns)
+               int *__restrict__ seed_cnt, int *__restrict__ seed_tiles, int turns, int tile)
 {{
     __shared__ int base;
     const int s = blockIdx.x;
@@ -111,7 +133,7 @@ ufo_bt_compact(const int *__restrict__ lost, const int *__restrict__ seed_off, i
     }}
     if (threadIdx.x == 0) {{
         seed_cnt[s] = base;
-        seed_tiles[s] = (base + {TILE - 1}) / {TILE};
+        seed_tiles[s] = (base + tile - 1) / tile;
     }}
 }}
 
@@ -165,6 +187,58 @@ ufo_bt_turns(ufloat *__restrict__ state, int *__restrict__ lost, const int *__re
         }}
         state[i] = x; state[np + i] = px; state[2 * np + i] = y; state[3 * np + i] = py;
         if (gone) lost[i] = turn;
+        }}
+    }}
+}}
+
+// two particles per lane: tiles of {2 * TILE}, lane l carries tile entries l and l + {TILE}
+extern "C" __global__ void __launch_bounds__({block})
+ufo_bt_turns2(ufloat *__restrict__ state, int *__restrict__ lost, const int *__restrict__ idx,
+              const int *__restrict__ seed_off, const int *__restrict__ seed_cnt, const int *__restrict__ tile_base,
+              const ufloat *__restrict__ table, int *__restrict__ counter, int seeds, int nparticles,
+              int turn0, int nturns, int turns)
+{{
+    {codegen._decls(env)}
+    ufloat xb, pxb, yb, pyb;
+    const int lane = __lane_id();
+    const size_t np = (size_t)nparticles;
+    const int ntiles = tile_base[seeds];
+    for (;;) {{
+        int w = 0;
+        if (lane == 0) w = atomicAdd(counter, 1);
+        w = __builtin_amdgcn_readfirstlane(__shfl(w, 0));
+        if (w >= ntiles) break;
+        int lo = 0, hi = seeds - 1;
+        while (lo < hi) {{
+            int mid = (lo + hi + 1) >> 1;
+            if (tile_base[mid] <= w) lo = mid; else hi = mid - 1;
+        }}
+        const int s = __builtin_amdgcn_readfirstlane(lo);
+        const int t = w - tile_base[s];
+        const int ka = t * {2 * TILE} + lane, kb = ka + {TILE};
+        if (ka < seed_cnt[s]) {{
+        const bool has_b = kb < seed_cnt[s];
+        const int ia = idx[seed_off[s] + ka];
+        const int ib = has_b ? idx[seed_off[s] + kb] : ia;
+        const ufloat *__restrict__ ct = table + (size_t)s * {nslots};
+{chr(10).join('        ' + x for x in init)}
+        x = state[ia]; px = state[np + ia]; y = state[2 * np + ia]; py = state[3 * np + ia];
+        xb = state[ib]; pxb = state[np + ib]; yb = state[2 * np + ib]; pyb = state[3 * np + ib];
+        int turn = turn0;
+        const int tend = (turn0 + nturns < turns) ? turn0 + nturns : turns;
+        int lost_a = -1, lost_b = has_b ? -1 : tend;
+        for (; turn < tend; ++turn) {{
+{chr(10).join('            ' + x_ for x_ in _paired(text.split(chr(10))))}
+            if (lost_a < 0 && (fabs(x) > {one} || fabs(y) > {one} || isnan(x))) lost_a = turn;
+            if (lost_b < 0 && (fabs(xb) > {one} || fabs(yb) > {one} || isnan(xb))) lost_b = turn;
+            if (lost_a >= 0 && lost_b >= 0) break;
+        }}
+        if (lost_a < 0) {{ state[ia] = x; state[np + ia] = px; state[2 * np + ia] = y; state[3 * np + ia] = py; }}
+        else lost[ia] = lost_a;
+        if (has_b) {{
+            if (lost_b < 0) {{ state[ib] = xb; state[np + ib] = pxb; state[2 * np + ib] = yb; state[3 * np + ib] = pyb; }}
+            else lost[ib] = lost_b;
+        }}
         }}
     }}
 }}
@@ -235,6 +309,8 @@ class BatchedAperture(Simulation):
         self.lost = self._lost_host.numpy().view(np.uint32)
         self.block_turns = block_turns
         self.waves_per_simd = None
+        self.particles_per_lane = 1
+        self.workgroup = self.kernel.block
         self._dev = None
         self._graphs = {}
         self._stream = None
@@ -267,67 +343,91 @@ class BatchedAperture(Simulation):
                 st[("x", "px", "y", "py").index(c), :self.particles] = b["pool"][:, self.coords.index(c)]
         b["lost"].fill_(self.turns)
 
-    def _grid(self, waves_per_simd):
+    def _grid(self, waves_per_simd, workgroup):
         lanes = self.ctx.device.compute_units * 4 * waves_per_simd * 64
-        block = self.kernel.block
-        return max(1, lanes // block)
+        return max(1, lanes // workgroup)
 
-    def _enqueue_block(self, turn0, nturns, waves_per_simd):
+    def _config(self):
+        return (self.block_turns, self.waves_per_simd, self.particles_per_lane, self.workgroup)
+
+    def _enqueue_block(self, turn0, nturns, waves_per_simd, ppl=1, workgroup=None):
+        """One block of turns: compaction (tiles of 64 * ppl particles), tile
+        scan, and the turn kernel with ``ppl`` particles per lane on a
+        persistent grid of ``waves_per_simd`` waves per SIMD in workgroups of
+        ``workgroup`` lanes."""
         b = self._buffers()
         mod = self.module()
+        wg = int(workgroup or self.kernel.block)
+        if ppl not in (1, 2) or wg % 64 or wg > self.kernel.block:
+            raise ValueError(f"unsupported launch shape: {ppl} particles per lane, workgroup {wg}")
         mod.launch("ufo_bt_compact", self.seeds, 256, [b["lost"], b["seed_off"], b["idx"], b["seed_cnt"],
-                                                         b["seed_tiles"], self.turns], stream=self._stream.cuda_stream)
+                                                         b["seed_tiles"], self.turns, TILE * ppl],
+                   stream=self._stream.cuda_stream)
         mod.launch("ufo_bt_scan", 1, 64, [b["seed_tiles"], b["tile_base"], b["counter"], self.seeds],
                    stream=self._stream.cuda_stream)
-        mod.launch("ufo_bt_turns", self._grid(waves_per_simd), self.kernel.block,
+        mod.launch("ufo_bt_turns" if ppl == 1 else "ufo_bt_turns2", self._grid(waves_per_simd, wg), wg,
                    [b["state"], b["lost"], b["idx"], b["seed_off"], b["seed_cnt"], b["tile_base"], b["table"],
                     b["counter"], self.seeds, self.particles, turn0, nturns, self.turns],
                    stream=self._stream.cuda_stream)
 
-    def _graph(self, block_turns, waves_per_simd):
-        key = (block_turns, waves_per_simd)
+    def _graph(self, block_turns, waves_per_simd, ppl=1, workgroup=None):
+        key = (block_turns, waves_per_simd, ppl, workgroup)
         g = self._graphs.get(key)
         if g is None:
             def seq():
                 for t0 in range(0, self.turns, block_turns):
-                    self._enqueue_block(t0, block_turns, waves_per_simd)
+                    self._enqueue_block(t0, block_turns, waves_per_simd, ppl, workgroup)
             g = runtime.capture(self._stream.cuda_stream, seq)
             self._graphs[key] = g
         return g
 
-    def tune(self, candidates=((32, 8), (64, 8), (128, 8), (64, 4)), sample_turns=256):
-        """Pick (turns per block, waves per SIMD) by timing the first
-        ``sample_turns`` turns of each candidate; cached per kernel and device."""
+    def _sample(self, bt, wps, ppl, wg, sample_turns):
+        self._reset()
+        torch.cuda.synchronize()
+        with torch.cuda.stream(self._stream):
+            t = time.perf_counter()
+            for t0 in range(0, min(sample_turns, self.turns), bt):
+                self._enqueue_block(t0, bt, wps, ppl, wg)
+            self._stream.synchronize()
+            return time.perf_counter() - t
+
+    def tune(self, candidates=((32, 8), (64, 8), (128, 8), (64, 4)), shapes=((1, 256), (2, 256), (1, 64), (2, 64)),
+             sample_turns=256):
+        """Pick turns per block, waves per SIMD, particles per lane and
+        workgroup size by timing the first ``sample_turns`` turns: first the
+        (turns per block, waves per SIMD) ``candidates`` with one particle per
+        lane in 256-lane workgroups, then the (particles per lane, workgroup)
+        ``shapes`` at the best pair.  Cached per kernel, workload and device."""
         key = hashlib.sha1((self.kernel.src + self.ctx.device.arch).encode()).hexdigest()[:16] + \
-            f"-{self.particles}-{self.turns}"
+            f"-{self.particles}-{self.turns}-v2"
         path = _tune_cache()
         try:
             cache = json.load(open(path))
         except (OSError, ValueError):
             cache = {}
-        if key in cache:
-            self.block_turns, self.waves_per_simd = cache[key]
-            return self.block_turns, self.waves_per_simd
+        if key in cache and len(cache[key]) == 4:
+            self.block_turns, self.waves_per_simd, self.particles_per_lane, self.workgroup = cache[key]
+            return self._config()
         self._stream = self._stream or torch.cuda.Stream(self.device)
         best = None
         for bt, wps in candidates:
-            self._reset()
-            torch.cuda.synchronize()
-            with torch.cuda.stream(self._stream):
-                t = time.perf_counter()
-                for t0 in range(0, min(sample_turns, self.turns), bt):
-                    self._enqueue_block(t0, bt, wps)
-                self._stream.synchronize()
-                dt = time.perf_counter() - t
+            dt = self._sample(bt, wps, 1, self.kernel.block, sample_turns)
             if best is None or dt < best[0]:
-                best = (dt, bt, wps)
-        self.block_turns, self.waves_per_simd = best[1], best[2]
-        cache[key] = [self.block_turns, self.waves_per_simd]
+                best = (dt, bt, wps, 1, self.kernel.block)
+        _, bt, wps, _, _ = best
+        for ppl, wg in shapes:
+            if (ppl, wg) == (1, self.kernel.block):
+                continue
+            dt = self._sample(bt, wps, ppl, wg, sample_turns)
+            if dt < best[0]:
+                best = (dt, bt, wps, ppl, wg)
+        _, self.block_turns, self.waves_per_simd, self.particles_per_lane, self.workgroup = best
+        cache[key] = list(self._config())
         try:
             json.dump(cache, open(path, "w"))
         except OSError:
             pass
-        return self.block_turns, self.waves_per_simd
+        return self._config()
 
     def launch(self):
         """Enqueue the whole run (the captured graph) on this object's stream,
@@ -338,7 +438,7 @@ class BatchedAperture(Simulation):
             else:
                 self.waves_per_simd = 8
         self._stream = self._stream or torch.cuda.Stream(self.device)
-        g = self._graph(self.block_turns, self.waves_per_simd)
+        g = self._graph(self.block_turns, self.waves_per_simd, self.particles_per_lane, self.workgroup)
         b = self._buffers()
         with torch.cuda.stream(self._stream):
             st = b["state"]
