@@ -1,0 +1,55 @@
+"""Host logic of the drivers that needs no GPU: parameter staging, kernel
+argument checks, launch-shape validation."""
+import numpy as np
+import pytest
+import torch
+
+import ufo_amd as ufo
+from ufo_amd import lattices, runtime
+from ufo_amd.sim import Simulation
+
+
+class _Stub(Simulation):
+    def __init__(self, shape):
+        self._init_common(ufo.DOUBLE_PRECISION, None, None)
+        self._pool_host = torch.zeros(shape, dtype=torch.float64)
+        self.parameters = self._pool_host.numpy()
+
+
+def test_parameters_filled_in_place_are_the_pool():
+    s = _Stub((4, 2))
+    s.parameters[:, 0] = [1, 2, 3, 4]
+    assert s._host_pool().numpy()[:, 0].tolist() == [1, 2, 3, 4]
+
+
+def test_rebound_parameters_are_copied_into_the_pool():
+    """The reference tracks whatever array ``parameters`` refers to at run()
+    (ufo/sa.py:141 copies self.parameters); rebinding the attribute works."""
+    s = _Stub((4, 2))
+    s.parameters = np.arange(8.0).reshape(4, 2)
+    assert np.array_equal(s._host_pool().numpy(), np.arange(8.0).reshape(4, 2))
+    s.parameters = np.arange(8.0)            # same size, flat
+    assert np.array_equal(s._host_pool().numpy(), np.arange(8.0).reshape(4, 2))
+    s.parameters = np.zeros((3, 2))
+    with pytest.raises(ValueError):
+        s._host_pool()
+
+
+def test_kernel_arguments_are_checked():
+    with pytest.raises(ValueError):
+        runtime.Module._args([torch.zeros(3)])                   # host tensor
+    with pytest.raises(ValueError):
+        runtime.Module._args([2 ** 31])                          # does not fit int
+    arr, holders = runtime.Module._args([5, 2.5, True])
+    assert len(holders) == 3
+
+
+def test_batched_launch_shape_validation():
+    from ufo_amd.batch import batch_kernels, _paired
+    ring, prm, vals, counts = lattices.dba_error_model(periods=4, seeds=2)
+    k, lay = batch_kernels(ring, ufo.KICK | ufo.DOUBLE_PRECISION | ufo.FIVED, ["x", "y"], prm)
+    assert "ufo_bt_turns2" in k.src and "ufo_bt_turns(" in k.src
+    # the paired body renames coordinates and temporaries, shares loop structure
+    out = _paired(["for (int rep = 0; rep < 4; ++rep) {", "x = fma(px, 0.5, x);", "{ ufloat t1 = x * y; }", "}"])
+    assert out == ["for (int rep = 0; rep < 4; ++rep) {", "x = fma(px, 0.5, x);", "xb = fma(pxb, 0.5, xb);",
+                   "{ ufloat t1 = x * y; }", "{ ufloat tb1 = xb * yb; }", "}"]

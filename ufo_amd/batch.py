@@ -335,7 +335,7 @@ class BatchedAperture(Simulation):
     def _reset(self):
         """State from the pool, everybody alive."""
         b = self._buffers()
-        b["pool"].copy_(self._pool_host, non_blocking=True)
+        b["pool"].copy_(self._host_pool(), non_blocking=True)
         st = b["state"]
         st.zero_()
         for c in ("x", "px", "y", "py"):
@@ -455,7 +455,7 @@ class BatchedAperture(Simulation):
         if self.block_turns is None:
             self._reset()
             self.tune()
-        b["pool"].copy_(self._pool_host)
+        b["pool"].copy_(self._host_pool())
         torch.cuda.synchronize()
         self.launch()
         self._stream.synchronize()

@@ -47,7 +47,7 @@ class ClosedOrbit(Simulation):
                          torch.empty(self._orbits_host.shape, dtype=self.dtype, device=self.device),
                          torch.empty(self._steps_host.shape, dtype=torch.int32, device=self.device))
         pool, orbits, steps = self._dev
-        pool.copy_(self._pool_host, non_blocking=True)
+        pool.copy_(self._host_pool(), non_blocking=True)
         if self.particles:
             block = self.kernel.block
             grid = (self.particles + block - 1) // block
@@ -91,7 +91,7 @@ class TangentMap(Simulation):
             self._dev = (torch.empty(self._pool_host.shape, dtype=self.dtype, device=self.device),
                          torch.empty(self._out_host.shape, dtype=self.dtype, device=self.device))
         pool, out = self._dev
-        pool.copy_(self._pool_host, non_blocking=True)
+        pool.copy_(self._host_pool(), non_blocking=True)
         if self.particles:
             block = self.kernel.block
             grid = (self.particles + block - 1) // block

@@ -13,6 +13,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <unordered_map>
 #include <vector>
@@ -153,10 +154,15 @@ int compile_rtc(const char *src, const char *options, const std::string &arch,
         return fail(UFO_E_COMPILE, "hiprtcCompileProgram: %s\n%s", hiprtcGetErrorString(r), log.c_str());
     }
     size_t size = 0;
-    hiprtcGetCodeSize(prog, &size);
+    r = hiprtcGetCodeSize(prog, &size);
+    if (r != HIPRTC_SUCCESS || size == 0) {
+        hiprtcDestroyProgram(&prog);
+        return fail(UFO_E_COMPILE, "hiprtcGetCodeSize: %s", hiprtcGetErrorString(r));
+    }
     code.resize(size);
-    hiprtcGetCode(prog, code.data());
+    r = hiprtcGetCode(prog, code.data());
     hiprtcDestroyProgram(&prog);
+    if (r != HIPRTC_SUCCESS) return fail(UFO_E_COMPILE, "hiprtcGetCode: %s", hiprtcGetErrorString(r));
     write_file_atomic(path, code.data(), code.size());
     return UFO_OK;
 }
@@ -177,7 +183,8 @@ struct ufo_ctx {
 struct ufo_module {
     int dev;
     hipModule_t mod;
-    std::unordered_map<std::string, hipFunction_t> funcs;
+    std::unordered_map<std::string, hipFunction_t> funcs;   // guarded by lock
+    std::mutex lock;
 };
 
 struct ufo_graph {
@@ -186,6 +193,7 @@ struct ufo_graph {
 };
 
 static int lookup(ufo_module *m, const char *kernel, hipFunction_t *f) {
+    std::lock_guard<std::mutex> guard(m->lock);
     auto it = m->funcs.find(kernel);
     if (it != m->funcs.end()) {
         *f = it->second;
@@ -282,7 +290,10 @@ int ufo_module_load(ufo_ctx *ctx, const void *code, size_t size, ufo_module **mo
     HIP_TRY(hipSetDevice(ctx->dev));
     hipModule_t m;
     HIP_TRY(hipModuleLoadData(&m, code));
-    *mod = new ufo_module{ctx->dev, m, {}};
+    ufo_module *um = new ufo_module;
+    um->dev = ctx->dev;
+    um->mod = m;
+    *mod = um;
     return UFO_OK;
 }
 
@@ -412,24 +423,21 @@ int ufo_graph_destroy(ufo_graph *graph) {
 int ufo_time_launch(ufo_module *mod, const char *kernel, const ufo_launch_cfg *cfg, void **args, size_t nargs,
                     void *stream, int reps, float *ms) {
     if (!ms || reps < 1) return fail(UFO_E_INVALID, "bad argument");
-    hipEvent_t a, b;
-    HIP_TRY(hipEventCreate(&a));
-    HIP_TRY(hipEventCreate(&b));
+    hipEvent_t a = nullptr, b = nullptr;
     hipStream_t s = static_cast<hipStream_t>(stream);
-    HIP_TRY(hipEventRecord(a, s));
-    for (int i = 0; i < reps; ++i) {
-        int rc = ufo_launch(mod, kernel, cfg, args, nargs, stream);
-        if (rc) {
-            (void)hipEventDestroy(a);
-            (void)hipEventDestroy(b);
-            return rc;
-        }
-    }
-    HIP_TRY(hipEventRecord(b, s));
-    HIP_TRY(hipEventSynchronize(b));
-    HIP_TRY(hipEventElapsedTime(ms, a, b));
-    (void)hipEventDestroy(a);
-    (void)hipEventDestroy(b);
+    int rc = UFO_OK;
+    hipError_t e = hipEventCreate(&a);
+    if (e == hipSuccess) e = hipEventCreate(&b);
+    if (e == hipSuccess) e = hipEventRecord(a, s);
+    for (int i = 0; e == hipSuccess && rc == UFO_OK && i < reps; ++i)
+        rc = ufo_launch(mod, kernel, cfg, args, nargs, stream);
+    if (e == hipSuccess && rc == UFO_OK) e = hipEventRecord(b, s);
+    if (e == hipSuccess && rc == UFO_OK) e = hipEventSynchronize(b);
+    if (e == hipSuccess && rc == UFO_OK) e = hipEventElapsedTime(ms, a, b);
+    if (a) (void)hipEventDestroy(a);
+    if (b) (void)hipEventDestroy(b);
+    if (rc) return rc;
+    if (e != hipSuccess) return fail(UFO_E_HIP, "event timing: %s", hipGetErrorString(e));
     return UFO_OK;
 }
 

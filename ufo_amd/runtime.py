@@ -220,13 +220,22 @@ class Module:
 
     @staticmethod
     def _args(args):
+        """Kernel argument array: device tensors by pointer (they must be
+        contiguous and on the GPU — the kernels index them densely), Python
+        ints as 32-bit ``int`` (range-checked), floats as ``double``."""
         holders = []
         for a in args:
             if isinstance(a, torch.Tensor):
+                if a.device.type != "cuda":
+                    raise ValueError(f"kernel argument on {a.device}: device tensors only")
+                if not a.is_contiguous():
+                    raise ValueError("kernel argument tensor is not contiguous")
                 holders.append(ctypes.c_void_p(a.data_ptr()))
             elif isinstance(a, bool):
                 holders.append(ctypes.c_int(int(a)))
             elif isinstance(a, int):
+                if not -2 ** 31 <= a < 2 ** 31:
+                    raise ValueError(f"integer kernel argument {a} does not fit a 32-bit int")
                 holders.append(ctypes.c_int(a))
             elif isinstance(a, float):
                 holders.append(ctypes.c_double(a))
@@ -299,12 +308,15 @@ class Graph:
 
 
 def capture(stream, fn_) -> Graph:
-    """Run ``fn_()`` under stream capture and return the instantiated graph."""
+    """Run ``fn_()`` under stream capture and return the instantiated graph
+    (capture always ends; a graph from a failed ``fn_`` is discarded)."""
     check(lib().ufo_graph_begin(ctypes.c_void_p(stream)), "graph begin")
+    h = ctypes.c_void_p()
     try:
         fn_()
-    finally:
-        h = ctypes.c_void_p()
-        rc = lib().ufo_graph_end(ctypes.c_void_p(stream), ctypes.byref(h))
-    check(rc, "graph end")
+    except BaseException:
+        if lib().ufo_graph_end(ctypes.c_void_p(stream), ctypes.byref(h)) == 0:
+            lib().ufo_graph_destroy(h)
+        raise
+    check(lib().ufo_graph_end(ctypes.c_void_p(stream), ctypes.byref(h)), "graph end")
     return Graph(h)

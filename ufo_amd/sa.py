@@ -78,7 +78,7 @@ class StableAperture(Simulation):
     def run(self, threads=1):
         """Copy ``parameters`` to the device, track, copy ``lost`` back."""
         self._buffers()
-        self._pool_dev.copy_(self._pool_host, non_blocking=True)
+        self._pool_dev.copy_(self._host_pool(), non_blocking=True)
         self.launch()
         self._lost_host.copy_(self._lost_dev, non_blocking=True)
         self.stream().synchronize()

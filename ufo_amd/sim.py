@@ -77,3 +77,24 @@ class Simulation:
 
     def stream(self):
         return torch.cuda.current_stream(self.device)
+
+    def _host_pool(self):
+        """The pinned pool tensor holding the current ``parameters``.  As in
+        the reference, whatever array ``parameters`` refers to at ``run()``
+        is what is tracked: filled in place (the usual form) it already is
+        the pinned buffer; if the attribute was rebound to another array, that
+        array is copied in (it must have the pool's shape)."""
+        host = self._pool_host
+        view = host.numpy()
+        p = self.parameters
+        same = (isinstance(p, np.ndarray) and p.dtype == view.dtype and p.shape == view.shape
+                and p.ctypes.data == view.ctypes.data)
+        if not same:
+            a = np.asarray(p, dtype=view.dtype)
+            if a.shape != view.shape:
+                if a.size == view.size:
+                    a = a.reshape(view.shape)
+                else:
+                    raise ValueError(f"parameters has shape {a.shape}; the pool is {view.shape}")
+            view[...] = a
+        return host

@@ -76,7 +76,7 @@ class Track(Simulation):
     def run(self, threads=1):
         """Copy ``parameters`` to the device, track, copy ``tracks`` back."""
         self._buffers()
-        self._pool_dev.copy_(self._pool_host, non_blocking=True)
+        self._pool_dev.copy_(self._host_pool(), non_blocking=True)
         self.launch()
         self._tracks_host.copy_(self._tracks_dev, non_blocking=True)
         self.stream().synchronize()
