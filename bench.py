@@ -289,22 +289,27 @@ def seed_adjoint_timing(seeds=64, per_seed=8192, turns=1024, every=32):
     seg = torch.tensor(seed_of, device=ba.device)
     torch.cuda.synchronize()
     stream = torch.cuda.current_stream()
-    ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
-    ev[0].record(stream)
-    xf, lost = ba.forward(x0)
-    ev[1].record(stream)
-    xf_t = xf.clone().requires_grad_(True)
-    alive = torch.isfinite(xf_t[:, 0])
-    r2 = torch.where(alive, xf_t[:, 0] ** 2 + xf_t[:, 2] ** 2, torch.zeros_like(xf_t[:, 0]))
-    pen = torch.where(alive, torch.nn.functional.softplus(20.0 * (r2 / 0.03 ** 2 - 0.25)) / 20.0,
-                      torch.ones_like(r2))
-    per = torch.zeros(seeds, dtype=torch.float64, device=ba.device).index_add_(0, seg, r2 + 1e-5 * pen) / per_seed
-    per.sum().backward()
-    ev[2].record(stream)
-    _, gseed = ba.backward(xf_t.grad)
-    ev[3].record(stream)
-    torch.cuda.synchronize()
-    f_ms, b_ms = ev[0].elapsed_time(ev[1]), ev[2].elapsed_time(ev[3])
+    fw, bw = [], []
+    for rnd in range(3):                    # one warm-up round, median of two
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+        ev[0].record(stream)
+        xf, lost = ba.forward(x0)
+        ev[1].record(stream)
+        xf_t = xf.clone().requires_grad_(True)
+        alive = torch.isfinite(xf_t[:, 0])
+        r2 = torch.where(alive, xf_t[:, 0] ** 2 + xf_t[:, 2] ** 2, torch.zeros_like(xf_t[:, 0]))
+        pen = torch.where(alive, torch.nn.functional.softplus(20.0 * (r2 / 0.03 ** 2 - 0.25)) / 20.0,
+                          torch.ones_like(r2))
+        per = torch.zeros(seeds, dtype=torch.float64, device=ba.device).index_add_(0, seg, r2 + 1e-5 * pen)
+        (per / per_seed).sum().backward()
+        ev[2].record(stream)
+        _, gseed = ba.backward(xf_t.grad)
+        ev[3].record(stream)
+        torch.cuda.synchronize()
+        if rnd:
+            fw.append(ev[0].elapsed_time(ev[1]))
+            bw.append(ev[2].elapsed_time(ev[3]))
+    f_ms, b_ms = float(np.median(fw)), float(np.median(bw))
     return {"workload": f"{seeds} seeds x {per_seed} particles, 4-period DBA with errors, 6D, {turns} turns, "
                         f"gradient w.r.t. {len(fam)} sextupole families per seed",
             "closed_orbit_ms": round(co_ms, 2), "newton_steps_max": int(co.steps.max()),
@@ -315,16 +320,19 @@ def seed_adjoint_timing(seeds=64, per_seed=8192, turns=1024, every=32):
             "hiprtc_compile_s": round(compile_s, 1), "gradient_finite": bool(torch.isfinite(gseed).all().item())}
 
 
-def adjoint_timing(cells=48, particles=65536, turns=512, every=32, reps=3):
+def adjoint_timing(cells=48, particles=65536, turns=512, every=32, warmup=2, reps=5):
     """Gradient of the amplitude loss with respect to all 96 quadrupole k1 and
     96 sextupole k2 of a 48-cell FODO ring (65 536 particles, sigma_x =
     sigma_y = 3 mm, 512 turns, checkpoints every 32 turns): time of the
-    checkpointing forward pass and of the reverse pass (HIP events on the
-    launch stream), and their ratio."""
+    checkpointing forward pass, of the reverse pass and of one plain forward
+    pass (the StableAperture kernel of the same ring and bunch, strengths
+    compiled in), HIP events on the launch stream.  Each round times the
+    three back to back; ``warmup`` rounds let the clock settle, the medians
+    of ``reps`` rounds are reported."""
     import numpy as np
     import torch
     import ufo_amd as ufo
-    from ufo_amd import adjoint, lattices
+    from ufo_amd import lattices
     ring = lattices.fodo_ring(cells, distinct=True)
     shared = ([(f"QF{i}", "k1") for i in range(cells)] + [(f"QD{i}", "k1") for i in range(cells)]
               + [(f"SF{i}", "k2") for i in range(cells)] + [(f"SD{i}", "k2") for i in range(cells)])
@@ -334,47 +342,40 @@ def adjoint_timing(cells=48, particles=65536, turns=512, every=32, reps=3):
                             every=every)
     th = theta.to(tr.device).requires_grad_(True)
     x0 = torch.tensor(bunch, device=tr.device)
+    sa = ufo.StableAperture(ring, flags=ufo.KICK | ufo.DOUBLE_PRECISION, turns=turns, particles=particles,
+                            parameters=["x", "px", "y", "py", "dp"])
+    sa.parameters[:] = bunch
+    sa.module()
+    sa.pool_device.copy_(sa._pool_host)
     stream = torch.cuda.current_stream()
-    fw, bw = [], []
-    for k in range(reps + 1):
-        e = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    fw, bw, pf = [], [], []
+    for k in range(warmup + reps):
+        e = [torch.cuda.Event(enable_timing=True) for _ in range(5)]
         e[0].record(stream)
         xf, lost = tr.forward(x0, th.detach())
         e[1].record(stream)
         lam = torch.zeros_like(xf)
         alive = torch.isfinite(xf[:, 0])
         lam[alive] = 2.0 * xf[alive] * torch.tensor([1.0, 0.0, 1.0, 0.0], device=tr.device) / particles
-        gx0, gth = tr.backward(lam)
         e[2].record(stream)
+        gx0, gth = tr.backward(lam)
+        e[3].record(stream)
+        sa.launch()
+        e[4].record(stream)
         torch.cuda.synchronize()
-        if k:
+        if k >= warmup:
             fw.append(e[0].elapsed_time(e[1]))
-            bw.append(e[1].elapsed_time(e[2]))
-    f_ms, b_ms = float(np.mean(fw)), float(np.mean(bw))
+            bw.append(e[2].elapsed_time(e[3]))
+            pf.append(e[3].elapsed_time(e[4]))
+    f_ms, b_ms, p_ms = float(np.median(fw)), float(np.median(bw)), float(np.median(pf))
     pt = int(torch.clamp(lost.to(torch.int64) + 1, max=turns).sum())
     flop = pt * tr.kfwd.flops_per_turn
-    # one plain forward pass of the same ring and bunch: the StableAperture
-    # kernel with the strengths compiled in (they equal theta)
-    sa = ufo.StableAperture(ring, flags=ufo.KICK | ufo.DOUBLE_PRECISION, turns=turns, particles=particles,
-                            parameters=["x", "px", "y", "py", "dp"])
-    sa.parameters[:] = bunch
-    sa.module()
-    sa.pool_device.copy_(sa._pool_host)
-    pf = []
-    for k in range(reps + 1):
-        e = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
-        e[0].record(stream)
-        sa.launch()
-        e[1].record(stream)
-        torch.cuda.synchronize()
-        if k:
-            pf.append(e[0].elapsed_time(e[1]))
-    p_ms = float(np.mean(pf))
     return {"workload": f"{cells}-cell FODO, {particles} particles, {turns} turns, checkpoints every {every}, "
                         f"gradient w.r.t. {len(shared)} strengths",
             "forward_ms": f_ms, "reverse_ms": b_ms, "gradient_over_forward": (f_ms + b_ms) / f_ms,
             "plain_forward_ms": p_ms, "gradient_over_plain_forward": (f_ms + b_ms) / p_ms,
-            "forward_tflops": flop / (f_ms * 1e-3) / 1e12, "survivors": int((lost == turns).sum())}
+            "forward_tflops": flop / (f_ms * 1e-3) / 1e12, "survivors": int((lost == turns).sum()),
+            "rounds": reps}
 
 
 def measured_traffic(args):

@@ -347,7 +347,7 @@ def _reverse_layout(ops, env: Env, ntheta: int):
             env.stride = None
         pre, top, end = codegen.prefetch_lines(env, str(reps - 1), "(rep > 0 ? rep - 1 : 0)")
         out += pre
-        out.append("#pragma clang loop unroll(disable)")
+        out.append(codegen.loop_pragma(env))
         out.append(f"for (int rep = {reps - 1}; rep >= 0; --rep) {{")
         out += ["    " + s for s in top + inner + end]
         out.append("}")
@@ -420,6 +420,9 @@ def adjoint_kernels(line, flags, turns, coords, lattice_parameters, dp=0.0, ever
     table: list = []
     slots: dict = {}
 
+    hd, pp, rp = codegen.find_period(ops) if len(ops) > 8 else (0, 0, 0)
+    unroll = codegen.unroll_for(ops[hd:hd + pp]) if rp >= 2 else 1
+
     def make_env():
         env = codegen._env_for(flags, src, None)
         env.shared_offset = len(coords)
@@ -427,6 +430,7 @@ def adjoint_kernels(line, flags, turns, coords, lattice_parameters, dp=0.0, ever
         env.table_name = "cs"
         env.prefetch = True
         env.factor_chroma = False
+        env.unroll = unroll
         return env
 
     P = max(1, len(coords))

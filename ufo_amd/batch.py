@@ -262,10 +262,11 @@ def build_table(layout, seed_params):
         full = np.concatenate([np.zeros(off), seed_params[s]])
         for k, e in enumerate(layout["flat"]):
             ct[s, k] = ir.evaluate(e, full)
+        maps = layout["stride"] if isinstance(layout["stride"], list) else None
         for j, e in enumerate(layout["period"]):
             for r in range(layout["reps"]):
-                ct[s, layout["F"] + j + r * layout["S"]] = ir.evaluate(
-                    ir.shift_params(e, r * layout["stride"], off), full)
+                er = ir.remap_params(e, maps[r]) if maps else ir.shift_params(e, r * layout["stride"], off)
+                ct[s, layout["F"] + j + r * layout["S"]] = ir.evaluate(er, full)
     return ct
 
 

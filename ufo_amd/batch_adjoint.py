@@ -43,8 +43,20 @@ def _table_env(flags, src, ncoord, diff_cols=None):
 
 
 def _layout_of(env, ncoord):
-    return {"flat": [e for e, _ in env.table], "period": [], "reps": 0, "stride": 0, "F": len(env.table), "S": 0,
-            "nslots": max(1, len(env.table)), "offset": ncoord}
+    """Table layout: flat slots, then the period slots of every repetition of
+    the period loop (if the turn has one), with the parameter stride or the
+    per-repetition column maps that turn repetition 0's slot expressions
+    into repetition r's."""
+    head, p, reps, stride = env.last_period
+    F, S = len(env.table), len(env.period_slots)
+    reps = int(reps) if reps >= 2 and S else 0
+    return {"flat": [e for e, _ in env.table], "period": sorted(env.period_slots, key=env.period_slots.get),
+            "reps": reps, "stride": stride if isinstance(stride, list) else int(stride), "F": F, "S": S,
+            "nslots": max(1, F + reps * S), "offset": ncoord}
+
+
+def _fill(text, layout):
+    return text.replace("@PB@", str(layout["F"])).replace("@PS@", str(max(layout["S"], 1)))
 
 
 def co_seed_kernel(line, flags, lattice_parameters, iterations=32, tol=1e-13, name="ufo_co_seed"):
@@ -53,8 +65,11 @@ def co_seed_kernel(line, flags, lattice_parameters, iterations=32, tol=1e-13, na
     ops = _ops(line, flags, coords + list(lattice_parameters))
     src = codegen._coord_sources(coords, 0, 0.0)
     env = _table_env(flags, src, 0)
+    env.no_loop = False
+    env.remap_loops = True
     body, _ = codegen._layout(ops, env, emit=tangent.emit_with_tangent)
     layout = _layout_of(env, 0)
+    body = [_fill(b, layout) for b in body]
     one, zero = env.lit(1.0), env.lit(0.0)
     code = codegen._PRELUDE.format(ufloat="double", nan='__builtin_nan("")') + "#define NT 4\n" + tangent._SOLVE4
     code += f"""
@@ -134,6 +149,79 @@ class SeedClosedOrbit(Simulation):
         return self.orbits
 
 
+def _reverse_turn(ops, env, diff_cols, period):
+    """Reverse pass of one turn for the batched adjoint: the ops backwards,
+    the repeated period as a descending loop (as the forward kernel's).
+    Gradient sums of a segment are flushed into the per-seed LDS
+    accumulators; inside the loop the accumulator of a sum depends on the
+    repetition (the parameters of repetition r are repetition 0's mapped by
+    the period's stride or column maps), looked up in a constant table
+    ``ufo_gmap[rep][k]``.  Returns (lines, gmap rows or None)."""
+    head, p, reps, stride = period
+    looped = reps >= 2
+
+    def image(i, r):
+        if isinstance(stride, list):
+            return stride[r].get(i, i)
+        return i + r * int(stride)
+
+    gmap_rows = [[] for _ in range(reps)] if looped else []
+
+    def segment(seg_ops, in_loop):
+        g = adjoint._Grad()
+        body: List[str] = []
+        for op in reversed(seg_ops):
+            adjoint.emit_reverse(op, env, body, g)
+        decl = [f"ufloat {r} = 0.0;" for r in g.sums.values()]
+        flush = []
+        for k, i in enumerate(g.params()):
+            terms = [(r, dd) for (ii, dd), r in g.sums.items() if ii == i]
+            acc_ = None
+            for r, dd in terms:
+                dv = env.val(dd)
+                acc_ = f"{r} * {dv}" if acc_ is None else f"fma({r}, {dv}, {acc_})"
+            if in_loop:
+                col = len(gmap_rows[0])
+                for r in range(reps):
+                    c = image(i, r)
+                    if c not in diff_cols:
+                        raise codegen.NeedsFlat("a repetition maps a gradient onto an undifferentiated column")
+                    gmap_rows[r].append(diff_cols.index(c))
+                idx = f"ufo_gmap[rep][{col}]"
+            else:
+                idx = str(diff_cols.index(i))
+            flush.append(f"unsafeAtomicAdd(&acc[{idx} * {ACCW} + slot], {acc_});")
+        return ["{"] + ["    " + x for x in decl + body + flush] + ["}"]
+
+    out: List[str] = []
+    if looped:
+        import copy
+        saved = copy.deepcopy((env.table, env.period_slots, env.invariants, env.inv_code, env.tmp))
+        try:
+            tail = segment(ops[head + p * reps:], False)
+            env.stride = ("rep", stride)
+            try:
+                inner = segment(ops[head:head + p], True)
+            finally:
+                env.stride = None
+            out += tail
+            out.append("#pragma clang loop unroll(disable)")
+            out.append(f"for (int rep = {reps - 1}; rep >= 0; --rep) {{")
+            out += ["    " + x for x in inner]
+            out.append("}")
+            out += segment(ops[:head], False)
+            return out, (gmap_rows if gmap_rows and gmap_rows[0] else None)
+        except codegen.NeedsFlat:
+            # restore the shared table and emit the turn flat
+            tb, ps, inv, code, tmp = saved
+            env.table[:] = tb
+            env.period_slots.clear()
+            env.period_slots.update(ps)
+            env.invariants, env.inv_code, env.tmp = inv, code, tmp
+            out = []
+    return segment(ops, False), None
+
+
 def batched_adjoint_kernels(line, flags, turns, coords, lattice_parameters, diff_parameters, dp=0.0, every=32):
     """Forward (checkpointing) and reverse kernels of the batched adjoint;
     ``diff_parameters`` is the subset of ``lattice_parameters`` whose
@@ -152,39 +240,24 @@ def batched_adjoint_kernels(line, flags, turns, coords, lattice_parameters, diff
     nd = max(1, len(diff_cols))
     pre = codegen._PRELUDE.format(ufloat="double", nan='__builtin_nan("")')
 
+    # one table for both kernels: the reverse env continues the forward's
     envf = _table_env(flags, src, off)
+    envf.no_loop = False
+    envf.remap_loops = True
     fbody, _ = codegen._layout(ops, envf)
     envb = _table_env(flags, src, off, diff_cols)
-    g = adjoint._Grad()
-    rbody: List[str] = []
-    for op in reversed(ops):
-        adjoint.emit_reverse(op, envb, rbody, g)
-    # both kernels share one table: the reverse kernel's slots follow the
-    # forward ones
-    table = list(envf.table)
-    for e, _ in envb.table:
-        if all(e != t for t, _ in table):
-            table.append((e, None))
-    remap = {}
-    for k, (e, _) in enumerate(envb.table):
-        remap[f"ct[{k}]"] = f"ct[{[t for t, _ in table].index(e)}]"
-    nslots = max(1, len(table))
-
-    def fix(lines):
-        import re
-        return [re.sub(r"ct\[(\d+)\]", lambda m: remap[m.group(0)], l_) for l_ in lines]
-
-    rbody = fix(rbody)
-    inv_b = fix(envb.inv_code)
-    decl = [f"ufloat {r} = 0.0;" for r in g.sums.values()]
-    flush = []
-    for i in g.params():
-        terms = [(r, dd) for (ii, dd), r in g.sums.items() if ii == i]
-        acc_ = None
-        for r, dd in terms:
-            dv = fix([envb.val(dd)])[0]
-            acc_ = f"{r} * {dv}" if acc_ is None else f"fma({r}, {dv}, {acc_})"
-        flush.append(f"unsafeAtomicAdd(&acc[{diff_cols.index(i)} * {ACCW} + slot], {acc_});")
+    envb.table, envb.period_slots = envf.table, envf.period_slots
+    rbody, gmaps = _reverse_turn(ops, envb, diff_cols, envf.last_period)
+    layout = _layout_of(envf, off)
+    nslots = layout["nslots"]
+    fbody = [_fill(x, layout) for x in fbody]
+    rbody = [_fill(x, layout) for x in rbody]
+    inv_b = [_fill(x, layout) for x in envb.inv_code]
+    decl, flush = [], []
+    gtab = ""
+    if gmaps:
+        rows = ", ".join("{" + ", ".join(str(v) for v in row) + "}" for row in gmaps)
+        gtab = f"static __constant__ int ufo_gmap[{len(gmaps)}][{len(gmaps[0])}] = {{{rows}}};\n"
 
     def load(env, inv, indent):
         lines = []
@@ -233,7 +306,7 @@ ufo_badj_fwd(const ufloat *__restrict__ pool, const ufloat *__restrict__ table, 
     lost[idx] = turn;
 }}
 """
-    bwd = pre + f"""
+    bwd = pre + gtab + f"""
 extern "C" __global__ void __launch_bounds__(64)
 ufo_badj_bwd(const ufloat *__restrict__ pool, const ufloat *__restrict__ table, const int *__restrict__ tile_seed,
              const int *__restrict__ tile_start, const int *__restrict__ tile_count, const ufloat *__restrict__ ckpt,
@@ -257,7 +330,7 @@ ufo_badj_bwd(const ufloat *__restrict__ pool, const ufloat *__restrict__ table, 
             const ufloat *c = ckpt + (size_t)ck * 4 * np + idx;
             x = c[0]; px = c[np]; y = c[2 * np]; py = c[3 * np];
             for (int turn = t_end - 1; turn >= t_begin; --turn) {{
-{chr(10).join('                ' + s_ for s_ in decl + rbody + flush)}
+{chr(10).join('                ' + s_ for s_ in rbody)}
             }}
         }}
         ufloat *o = gx0 + (size_t)idx * 4;
@@ -271,8 +344,6 @@ ufo_badj_bwd(const ufloat *__restrict__ pool, const ufloat *__restrict__ table, 
     }}
 }}
 """
-    layout = {"flat": [e for e, _ in table], "period": [], "reps": 0, "stride": 0, "F": len(table), "S": 0,
-              "nslots": nslots, "offset": off}
     fk = codegen.Kernel(name="ufo_badj_fwd", src=fwd, kind="badj_fwd", fp32=False,
                         flops_per_turn=codegen._count(ops, envf), elements=codegen.element_count(line), block=64)
     bk = codegen.Kernel(name="ufo_badj_bwd", src=bwd, kind="badj_bwd", fp32=False,

@@ -60,6 +60,10 @@ def literal(v: float, fp32: bool) -> str:
     return f"{s}f" if fp32 else s
 
 
+class NeedsFlat(Exception):
+    """A period loop cannot be emitted for this op list; emit it flat."""
+
+
 @dataclass
 class Env:
     """Code-generation context of one kernel."""
@@ -81,6 +85,8 @@ class Env:
     prefetch: bool = False           # period-loop coefficients in registers K<j>, loaded one repetition ahead
     factor_chroma: bool = True       # table mode: apply 1/(1+dp) to whole kick sums
     products: Dict[tuple, str] = field(default_factory=dict)  # prefetch mode: (slot, variable) -> register
+    remap_loops: bool = False        # table mode: loop over repetitions whose parameters map column to column
+    unroll: int = 1                  # period-loop unroll factor (1: the loop stays rolled)
 
     def f(self, e) -> Expr:
         return fold(e, self.fp32, self.consts)
@@ -94,6 +100,13 @@ class Env:
         k = t[0]
         if k == "c":
             return self.lit(t[1])
+        if k == "p" and self.stride is not None and isinstance(self.stride[1], list):
+            if t[1] >= self.shared_offset:
+                # inside a remapped period loop every lattice parameter must
+                # come from the coefficient table
+                raise NeedsFlat(f"parameter column {t[1]} read directly in a remapped loop")
+            self.params_used.add(t[1])
+            return f"P{t[1]}"
         if k == "p" and t[1] >= self.shared_offset:
             j = t[1] - self.shared_offset
             if self.stride is not None and self.stride[1] != 0:
@@ -618,7 +631,26 @@ def _param_stride(par, head, p, reps):
     return 0 if stride is None else stride
 
 
-def find_period(ops, with_stride=False):
+def _param_maps(par, head, p, reps, fixed_below):
+    """Per-repetition maps of parameter columns (repetition 0 -> r), or None
+    when a column would map to two columns or a column below
+    ``fixed_below`` (per-particle coordinates) would move."""
+    maps = [dict() for _ in range(reps)]
+    for r in range(reps):
+        m = maps[r]
+        for j in range(p):
+            a, b = par[head + j], par[head + r * p + j]
+            if len(a) != len(b):
+                return None
+            for ia, ib in zip(a, b):
+                if ia < fixed_below and ia != ib:
+                    return None
+                if m.setdefault(ia, ib) != ib:
+                    return None
+    return maps
+
+
+def find_period(ops, with_stride=False, remap_from=None):
     """Largest coverage (head, period, repeats[, stride]) of the op list by a
     repeated block starting within the first few ops.  Repetitions may
     differ in their parameter indices by a constant stride per repetition
@@ -651,9 +683,13 @@ def find_period(ops, with_stride=False):
     for cov, _, _, head, p, reps in cands:
         if cov <= best_cov:
             break
-        # shrink the repeat count until the parameter indices follow a stride
+        # shrink the repeat count until the parameter indices follow a
+        # stride (or, with remap_from set, a column-to-column map per
+        # repetition; columns below remap_from stay fixed)
         while reps >= 2 and reps * p > best_cov:
             st = _param_stride(par, head, p, reps)
+            if st is None and remap_from is not None:
+                st = _param_maps(par, head, p, reps, remap_from)
             if st is not None:
                 best, best_cov = ((head, p, reps, st) if with_stride else (head, p, reps)), reps * p
                 break
@@ -703,7 +739,21 @@ def _layout(body_ops, env, observe=None, loop_var="rep", emit=None):
     one.  ``emit(op, env, out, observe)`` writes one op (default: the
     forward pass)."""
     emit = emit or emit_op
-    head, p, reps, stride = find_period(body_ops, True) if len(body_ops) > 8 and not env.no_loop else (0, 0, 0, 0)
+    remap_from = env.shared_offset if (env.remap_loops and env.table is not None) else None
+    head, p, reps, stride = (find_period(body_ops, True, remap_from) if len(body_ops) > 8 and not env.no_loop
+                             else (0, 0, 0, 0))
+    if reps >= 2 and isinstance(stride, list):
+        # a remapped loop works only if every lattice parameter of the
+        # period reaches the table: try on a copy, fall back to flat
+        import copy
+        trial = copy.deepcopy(env)
+        trial.remap_loops = False
+        try:
+            trial.stride = ("rep", stride)
+            for op in body_ops[head:head + p]:
+                emit(op, trial, [], observe)
+        except NeedsFlat:
+            head, p, reps, stride = 0, 0, 0, 0
     env.last_period = (head, p, reps, stride)
     out: List[str] = []
     if reps >= 2:
@@ -719,7 +769,7 @@ def _layout(body_ops, env, observe=None, loop_var="rep", emit=None):
         nxt = f"({loop_var} + 1 < {reps} ? {loop_var} + 1 : {loop_var})"
         pre, top, end = prefetch_lines(env, "0", nxt)
         out += pre
-        out.append("#pragma clang loop unroll(disable)")
+        out.append(loop_pragma(env))
         out.append(f"for (int {loop_var} = 0; {loop_var} < {reps}; ++{loop_var}) {{")
         out += ["    " + s for s in top + inner + end]
         out.append("}")
@@ -729,6 +779,23 @@ def _layout(body_ops, env, observe=None, loop_var="rep", emit=None):
         for op in body_ops:
             emit(op, env, out, observe)
     return out, (head, p, reps)
+
+
+def loop_pragma(env: Env) -> str:
+    """Unrolling of the period loop: off by default (the loop body of a ring
+    period already fills the instruction stream); a small body may be
+    unrolled ``env.unroll`` times so the scheduler interleaves neighbouring
+    repetitions."""
+    if env.unroll > 1:
+        return f"#pragma clang loop unroll_count({int(env.unroll)})"
+    return "#pragma clang loop unroll(disable)"
+
+
+def unroll_for(period_ops, budget=1024, most=4) -> int:
+    """Unroll factor for a period loop: as many repetitions (up to ``most``)
+    as fit in ``budget`` canonical FLOPs."""
+    f = sum(ir.op_flops(op) for op in period_ops) if period_ops else budget
+    return int(max(1, min(most, budget // max(f, 1))))
 
 
 def prefetch_lines(env: Env, first: str, nxt: str):

@@ -218,6 +218,12 @@ def shift_params(e: Expr, offset: int, start: int = 0) -> Expr:
                       else l)
 
 
+def remap_params(e: Expr, mapping) -> Expr:
+    """Copy of ``e`` with parameter column i replaced by mapping[i] (columns
+    not in the mapping stay)."""
+    return map_leaves(Expr.of(e), lambda l: Expr(("p", mapping.get(l.t[1], l.t[1]))) if l.t[0] == "p" else l)
+
+
 def select(cond, a, b) -> Expr:
     """``cond ? a : b`` (cond is a comparison expression)."""
     return Expr(("?", Expr.of(cond), Expr.of(a), Expr.of(b)))
