@@ -165,6 +165,8 @@ def emit_kick_reverse(op: ir.Kick, env: Env, out: List[str], g: _Grad):
     for i, (cr, ci) in enumerate(d):
         if not is_zero(cr) or not is_zero(ci):
             top = i
+    if top == 2 and all(is_zero(c) for c in (d[0][0], d[0][1], d[1][0], d[1][1])):
+        return _emit_quadratic_reverse(op, d, env, out, g)
     lines: List[str] = []
 
     def v(e):
@@ -240,6 +242,41 @@ def emit_kick_reverse(op: ir.Kick, env: Env, out: List[str], g: _Grad):
         lines.append(f"lx = {lx}; ly = {ly};")
     if lines:
         out.append("{ " + " ".join(lines) + " }")
+
+
+def _emit_quadratic_reverse(op: ir.Kick, d, env: Env, out: List[str], g: _Grad):
+    """Reverse of a pure second-order kick (a sextupole slice, w = d2 z^2):
+    z^2 is formed once and serves both the inverse kick (px += Re d2 z^2,
+    py -= Im d2 z^2) and the coefficient adjoint; lambda <- J^T lambda with
+    w' = 2 d2 z.  14 operations for a normal sextupole slice instead of the
+    18 of the Horner form."""
+    d2r = None if is_zero(d[2][0]) else env.val(d[2][0])
+    d2i = None if is_zero(d[2][1]) else env.val(d[2][1])
+    zr, zi = env.new_tmp(), env.new_tmp()
+    lines = [f"ufloat {zr} = fma(x, x, -(y * y)); ufloat {zi} = fma(x, y, y * x);"]
+    px, py = "px", "py"
+    if d2r is not None:
+        px = _fma(d2r, zr, px)
+        py = _fma_neg(d2r, zi, py)
+    if d2i is not None:
+        px = _fma_neg(d2i, zi, px)
+        py = _fma_neg(d2i, zr, py)
+    lines.append(f"px = {px}; py = {py};")
+    if any(ir.Expr.of(c).params() for c in list(op.kn) + list(op.ks)):
+        chi = ir.OODPPO if op.chroma else ir.E(1.0)
+        scale = chi / 2.0
+        if len(op.kn) > 2 and ir.Expr.of(op.kn[2]).params():
+            _add_grad(env, g, op.kn[2] * scale, [("-lpx", zr), ("lpy", zi)], lines)
+        if len(op.ks) > 2 and ir.Expr.of(op.ks[2]).params():
+            _add_grad(env, g, op.ks[2] * scale, [("lpx", zi), ("lpy", zr)], lines)
+    dr = None if is_zero(d[2][0]) else env.val(env.f(d[2][0] * 2.0))
+    di = None if is_zero(d[2][1]) else env.val(env.f(d[2][1] * 2.0))
+    a, b = env.new_tmp(), env.new_tmp()
+    nr = codegen._chain([(dr, "x", False), (di, "y", True)], None)
+    ni = codegen._chain([(dr, "y", False), (di, "x", False)], None)
+    lines.append(f"ufloat {a} = {nr}; ufloat {b} = {ni};")
+    lines.append(f"lx = {_fma(b, 'lpy', _fma_neg(a, 'lpx', 'lx'))}; ly = {_fma(b, 'lpx', _fma(a, 'lpy', 'ly'))};")
+    out.append("{ " + " ".join(lines) + " }")
 
 
 def emit_kick_reverse_factored(op: ir.Kick, env: Env, out: List[str], g: _Grad):
