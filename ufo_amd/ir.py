@@ -640,8 +640,3 @@ def op_flops(op: Op) -> int:
     return 0
 
 
-def count_flops(ops, fp32=False, env=None) -> int:
-    total = 0
-    for op in ops:
-        total += op_flops(op.map_exprs(lambda e: fold(e, fp32, env)))
-    return total
