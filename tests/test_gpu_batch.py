@@ -2,6 +2,8 @@
 the oracle, where each particle carries its seed's error values as
 per-particle parameters (the reference's way of batching lattice variants,
 HOWTO.md:20-22)."""
+import os
+
 import numpy as np
 import pytest
 
@@ -9,6 +11,8 @@ import ufo_amd as ufo
 from ufo_amd import lattices
 from ufo_amd.batch import BatchedAperture
 from oracle import runner
+
+from conftest import OPTICS_DIR
 
 pytestmark = pytest.mark.gpu
 
@@ -37,6 +41,39 @@ def test_batched_matches_oracle_per_particle_parameters():
           f"differ {np.sum(lost != strict)}")
     assert 0.2 * len(pool) < np.sum(strict < turns) < 0.9 * len(pool)
     assert np.array_equal(lost[regular], strict[regular])
+
+
+def test_batched_thick_alba_per_seed_quadrupoles():
+    """Thick elements (no KICK): three variants of ALBA with per-seed QF1 and
+    QD2 gradients (the thick matrices' square roots and trigonometric
+    functions of the per-seed strengths live in the seed table), FIVED, 300
+    turns over the HOWTO's 12 x 12 grid; loss turns equal the oracle's on
+    every regular particle."""
+    alba = ufo.Lattice(os.path.join(OPTICS_DIR, "alba.mad"))
+    flags = ufo.DOUBLE_PRECISION | ufo.FIVED
+    prm = [("QF1", "k1"), ("QD2", "k1")]
+    qf1 = [e.k1 for e in alba.RING.flatten() if e.label == "QF1"][0]
+    qd2 = [e.k1 for e in alba.RING.flatten() if e.label == "QD2"][0]
+    vals = np.array([[qf1, qd2], [qf1 * 1.01, qd2], [qf1, qd2 * 0.99]])
+    xx, yy = np.meshgrid(np.linspace(-0.04, 0.04, 12), np.linspace(-0.04, 0.04, 12))
+    grid = np.stack([xx.ravel(), yy.ravel()], 1)
+    counts = np.array([len(grid)] * 3)
+    turns = 300
+    ba = BatchedAperture(alba.RING, flags, turns, prm, vals, counts, block_turns=64)
+    pool = np.concatenate([grid] * 3)
+    ba.parameters[:] = pool
+    ba.run()
+    seed_of = np.repeat(np.arange(3), counts)
+    full = np.concatenate([pool, vals[seed_of]], axis=1)
+    strict = runner.stable_aperture(alba.RING, full, flags=flags, turns=turns, parameters=["x", "y"] + prm)
+    fast = runner.stable_aperture(alba.RING, full, flags=flags, turns=turns, parameters=["x", "y"] + prm, mode="fast")
+    regular = strict == fast
+    lost = ba.lost.astype(np.int64)
+    print(f"lost {np.sum(strict < turns)} of {len(pool)}, regular {regular.sum()}, differ {np.sum(lost != strict)}")
+    assert np.array_equal(lost[regular], strict[regular])
+    assert 0 < np.sum(strict < turns) < len(pool) and regular.mean() > 0.8
+    # the variants differ: the seeds' tables are used
+    assert not np.array_equal(strict[:144], strict[144:288]) or not np.array_equal(strict[:144], strict[288:])
 
 
 def test_graph_replay_and_launch_shape_do_not_change_results():
