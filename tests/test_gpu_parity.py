@@ -64,6 +64,22 @@ def test_fodo_track_6d():
     assert np.all(gpu[alive, :, 4] == 0.0)
 
 
+def test_fodo_config0_full_size():
+    """The reference's own CPU workload at full size: FODO-16, 4096
+    particles (seed 0), 6D, 1000 turns, where=[-1].  After 1000 turns the
+    reference's arithmetic disagrees with itself at the 4e-12 level (median
+    over particles of its fast-math and one-ulp-moved runs), so few
+    particles are regular: the bar is identical loss flags, 1e-12 on the
+    regular ones, and the statistical comparison on all."""
+    ring = lattices.fodo_ring(16)
+    bunch = lattices.gaussian_bunch(4096, seed=0)
+    gpu, strict, fast, moved = _track_both(ring, bunch, ufo.KICK | ufo.DOUBLE_PRECISION, 1000, COORD6, [-1])
+    r = compare_tracks(gpu, strict, fast, moved)
+    print(summary(r))
+    assert_parity(r, min_regular_frac=0.0)
+    assert np.array_equal(np.isnan(gpu[:, :, 0]), np.isnan(strict[:, :, 0]))
+
+
 def test_dba_stable_aperture_subset():
     """DA ring, a 32x8 subset of the grid (including the lossy corner),
     1024 turns: identical loss turns on regular particles."""
