@@ -281,6 +281,28 @@ def test_optics_periodic_equals_propagated():
         assert abs(getattr(per, a)[-1] - getattr(pro, a)[-1]) < 1e-9 * max(1.0, abs(getattr(per, a)[-1]))
 
 
+def test_optics_chromaticity_by_dp_parameter():
+    """tests/optics3.py on the HIP Optics: dp as a kernel parameter, LINEAR
+    (chromatic, no sextupoles), tune shift over dp = 1e-4 against MAD-X's
+    natural chromaticity (-40.13, -27.05) within 0.1, and equal to the
+    oracle's finite difference to 1e-6."""
+    alba = ufo.Lattice(os.path.join(OPTICS_DIR, "alba.mad"))
+    fl = ufo.LINEAR | ufo.DOUBLE_PRECISION
+    res = []
+    for make in (lambda: ufo.Optics(alba.RING, where=[], parameters=["dp"], flags=fl),
+                 lambda: runner.Optics(alba.RING, where=[], parameters=["dp"], flags=fl)):
+        o = make()
+        o.parameters[0] = 0.0
+        o.run() if isinstance(o, runner.Optics) else o.run(threads=1)
+        q0 = (o.qx, o.qy)
+        o.parameters[0] = 1e-4
+        o.run() if isinstance(o, runner.Optics) else o.run(threads=1)
+        res.append(((o.qx - q0[0]) / 1e-4, (o.qy - q0[1]) / 1e-4))
+    print("dQ/ddp hip", res[0], "oracle", res[1])
+    assert abs(res[0][0] + 40.13060686) < 0.1 and abs(res[0][1] + 27.05066939) < 0.1
+    assert np.allclose(res[0], res[1], rtol=1e-6)
+
+
 def test_sa_threads_and_refill():
     """Results do not depend on the number of resident lanes (persistent
     refill) — run with 64 lanes and with the full device."""

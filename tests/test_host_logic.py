@@ -53,3 +53,15 @@ def test_batched_launch_shape_validation():
     out = _paired(["for (int rep = 0; rep < 4; ++rep) {", "x = fma(px, 0.5, x);", "{ ufloat t1 = x * y; }", "}"])
     assert out == ["for (int rep = 0; rep < 4; ++rep) {", "x = fma(px, 0.5, x);", "xb = fma(pxb, 0.5, xb);",
                    "{ ufloat t1 = x * y; }", "{ ufloat tb1 = xb * yb; }", "}"]
+
+
+def test_fp32_host_evaluation_rounds_every_step():
+    """ir.evaluate(fp32=True) reproduces an FP32 kernel's evaluation of the
+    same expression: every leaf and operation rounded to float32."""
+    from ufo_amd import ir
+    e = ir.Expr.of("parameters[0]") * 0.3 / 7.0 + ir.Expr.of("parameters[1]") * ir.Expr.of("parameters[0]")
+    p = [1.2345678901234567, -0.987654321]
+    f32 = np.float32
+    expect = f32(f32(f32(p[0]) * f32(0.3)) / f32(7.0)) + f32(f32(p[1]) * f32(p[0]))
+    assert ir.evaluate(e, p, None, True) == float(expect)
+    assert ir.evaluate(e, p) == (p[0] * 0.3 / 7.0 + p[1] * p[0])

@@ -251,7 +251,7 @@ ufo_bt_turns2(ufloat *__restrict__ state, int *__restrict__ lost, const int *__r
     return k, layout
 
 
-def build_table(layout, seed_params):
+def build_table(layout, seed_params, fp32=False):
     """Per-seed coefficient table [seeds, nslots] from the per-seed values of
     the lattice parameters [seeds, P] (parameter column offset: the
     coordinates come first)."""
@@ -261,12 +261,12 @@ def build_table(layout, seed_params):
     for s in range(seeds):
         full = np.concatenate([np.zeros(off), seed_params[s]])
         for k, e in enumerate(layout["flat"]):
-            ct[s, k] = ir.evaluate(e, full)
+            ct[s, k] = ir.evaluate(e, full, None, fp32)
         maps = layout["stride"] if isinstance(layout["stride"], list) else None
         for j, e in enumerate(layout["period"]):
             for r in range(layout["reps"]):
                 er = ir.remap_params(e, maps[r]) if maps else ir.shift_params(e, r * layout["stride"], off)
-                ct[s, layout["F"] + j + r * layout["S"]] = ir.evaluate(er, full)
+                ct[s, layout["F"] + j + r * layout["S"]] = ir.evaluate(er, full, None, fp32)
     return ct
 
 
@@ -301,7 +301,7 @@ class BatchedAperture(Simulation):
         self.seeds = len(self.counts)
         self.seed_values = np.asarray(seed_values, dtype=np.float64).reshape(self.seeds, -1)
         self.kernel, self.layout = batch_kernels(line, flags, self.coords, self.lattice_parameters, dp)
-        self.table = build_table(self.layout, self.seed_values)
+        self.table = build_table(self.layout, self.seed_values, self.fp32)
         self.particles = int(self.counts.sum())
         self.offsets = np.concatenate([[0], np.cumsum(self.counts)]).astype(np.int32)
         self._pool_host = host_buffer((self.particles, max(1, len(self.coords))), self.dtype)

@@ -184,32 +184,35 @@ def diff(e: Expr, i: int) -> Expr:
     raise ValueError(f"cannot differentiate {e!r}")
 
 
-def evaluate(e: Expr, params, env=None) -> float:
-    """Value of ``e`` in FP64 on the host with parameter column i bound to
-    ``params[i]`` and kernel variables to ``env``."""
+def evaluate(e: Expr, params, env=None, fp32=False) -> float:
+    """Value of ``e`` on the host with parameter column i bound to
+    ``params[i]`` and kernel variables to ``env``, in FP64 or, with
+    ``fp32``, with every leaf and every operation rounded to FP32 (the
+    arithmetic of an FP32 kernel evaluating the same expression)."""
     t = Expr.of(e).t
     k = t[0]
+    r32 = (lambda v: float(np.float32(v))) if fp32 else (lambda v: v)
     if k == "c":
-        return t[1]
+        return r32(t[1])
     if k == "p":
-        return float(params[t[1]])
+        return r32(float(params[t[1]]))
     if k == "v":
-        return float(env[t[1]])
+        return r32(float(env[t[1]]))
     if k == "r":
         raise ValueError("raw C text cannot be evaluated on the host")
     if k == "neg":
-        return -evaluate(t[1], params, env)
+        return -evaluate(t[1], params, env, fp32)
     if k == "f":
-        return _LIBM[t[1]](evaluate(t[2], params, env))
+        return r32(_LIBM[t[1]](evaluate(t[2], params, env, fp32)))
     if k == "?":
-        return evaluate(t[2] if evaluate(t[1], params, env) != 0.0 else t[3], params, env)
-    a = evaluate(t[1], params, env)
-    b = evaluate(t[2], params, env)
+        return evaluate(t[2] if evaluate(t[1], params, env, fp32) != 0.0 else t[3], params, env, fp32)
+    a = evaluate(t[1], params, env, fp32)
+    b = evaluate(t[2], params, env, fp32)
     if k == ">":
         return float(a > b)
     if k == "<":
         return float(a < b)
-    return _binop(k, a, b, False)
+    return _binop(k, a, b, fp32)
 
 
 def shift_params(e: Expr, offset: int, start: int = 0) -> Expr:
