@@ -14,6 +14,14 @@ def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X GPU (HIP device)")
 
 
+def pytest_sessionstart(session):
+    """Build libufo_hip.so once if nothing is built yet (a fresh checkout);
+    a library already in the tree is used as it is."""
+    from ufo_amd import build
+    if not os.path.exists(build.LIB):
+        build.build_library()
+
+
 def gpu_available():
     try:
         import torch
