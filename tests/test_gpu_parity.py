@@ -355,3 +355,76 @@ def test_alba_stable_aperture_howto_grid(double):
         neigh = [pad[1:-1, :-2], pad[1:-1, 2:], pad[:-2, 1:-1], pad[2:, 1:-1]]
         interior = (ref == 0) | ((ref == 1000) & np.all([nb == 1000 for nb in neigh], axis=0))
         assert np.all(sa.lost[interior.ravel()] == ref.ravel()[interior.ravel()])
+
+
+def test_reference_aperture_case():
+    """tests/aperture.py on the HIP Track: one Aperture('ap', window='x > 0.1'),
+    FIVED, FP32 (the reference test's flags), x = 0 and x = 0.2.  The window
+    is spliced as ``if (x > 0.1 == 0) x = NAN;`` (ufo/elements.py:518), so x = 0
+    is lost and x = 0.2 kept, as in the oracle; a two-term window binds
+    ``a && b == 0`` as ``a && (b == 0)``."""
+    line = ufo.Line("a_line")
+    line.append(ufo.Aperture("ap", window="x > 0.1"))
+    tr = ufo.Track(line, turns=1, particles=2, where=[-1], flags=ufo.FIVED, parameters=["x"])
+    tr.parameters[0, 0] = 0.0
+    tr.parameters[1, 0] = 0.2
+    tr.run(threads=1)
+    ref = runner.track(line, tr.parameters.copy(), flags=ufo.FIVED, turns=1, parameters=["x"], where=[-1])
+    assert np.isnan(tr.tracks[0, 0, 0]) and tr.tracks[1, 0, 0] == np.float32(0.2)
+    np.testing.assert_array_equal(tr.tracks[:, :, :4], ref[:, :, :4])
+
+    line = ufo.Line("l", [ufo.Aperture("ap", window="x > 0.1 && y > 0.1")])
+    pool = np.array([[0.2, 0.0], [0.2, 0.2], [0.0, 0.0]])
+    fl = ufo.FIVED | ufo.DOUBLE_PRECISION
+    tr = ufo.Track(line, turns=1, particles=3, where=[-1], flags=fl, parameters=["x", "y"])
+    tr.parameters[:] = pool
+    tr.run()
+    assert np.isnan(tr.tracks[0, 0, 0]) and tr.tracks[1, 0, 0] == 0.2 and tr.tracks[2, 0, 0] == 0.0
+
+
+def test_reference_align_case():
+    """tests/align.py on the HIP Optics: the FODO with QF displaced by
+    dx = 1e-4 has the dispersion of the line whose QF is a combined bend of
+    angle 1e-4 (optics/align.mad), within 1e-6, in the reference test's FP32
+    and in FP64; each equals the oracle's Optics."""
+    for fl, tol in [(ufo.LINEAR | ufo.ACHROMATIC, 1e-6),
+                    (ufo.LINEAR | ufo.ACHROMATIC | ufo.DOUBLE_PRECISION, 1e-12)]:
+        lat = ufo.Lattice(os.path.join(OPTICS_DIR, "align.mad"))
+        lat.QF.dx = 1e-4
+        fodo = ufo.Optics(lat.FODO, where=[-1], flags=fl)
+        equivalent = ufo.Optics(lat.EQUIVALENT, where=[-1], flags=fl)
+        fodo.run(threads=1)
+        equivalent.run(threads=1)
+        assert abs(fodo.dx[0] - equivalent.dx[0]) < 1e-6
+        for o, line in [(fodo, lat.FODO), (equivalent, lat.EQUIVALENT)]:
+            r = runner.Optics(line, where=[-1], flags=fl)
+            r.run()
+            assert abs(o.dx[0] - r.dx[0]) <= tol * max(1.0, abs(r.dx[0]))
+            assert abs(o.qx - r.qx) <= tol and abs(o.qy - r.qy) <= tol
+
+
+def test_reference_thick_vs_kick_case():
+    """tests/thick_vs_kick.py on the HIP Optics: ALBA with 16 quadrupole and
+    48 bend slices (set through ufo.DEFAULT_QUADRUPOLE_SLICES /
+    DEFAULT_BEND_SLICES before the lattice is loaded), teapot and thick tunes
+    agree within 5e-4; in FP64 each tune equals the oracle's to 1e-10."""
+    saved = ufo.DEFAULT_QUADRUPOLE_SLICES, ufo.DEFAULT_BEND_SLICES
+    try:
+        ufo.DEFAULT_QUADRUPOLE_SLICES = 16
+        ufo.DEFAULT_BEND_SLICES = 48
+        alba = ufo.Lattice(os.path.join(OPTICS_DIR, "alba.mad"))
+    finally:
+        ufo.DEFAULT_QUADRUPOLE_SLICES, ufo.DEFAULT_BEND_SLICES = saved
+    assert all(e.slices == 16 for e in alba.RING.flatten() if isinstance(e, ufo.Quadrupole))
+    for double in (False, True):
+        fl = ufo.LINEAR | ufo.ACHROMATIC | (ufo.DOUBLE_PRECISION if double else 0)
+        kick = ufo.Optics(alba.RING, flags=fl | ufo.KICK)
+        thick = ufo.Optics(alba.RING, flags=fl)
+        kick.run(threads=1)
+        thick.run(threads=1)
+        assert abs(thick.qx - kick.qx) < 5e-4 and abs(thick.qy - kick.qy) < 5e-4
+        if double:
+            for o, f in [(kick, fl | ufo.KICK), (thick, fl)]:
+                r = runner.Optics(alba.RING, flags=f)
+                r.run()
+                assert abs(o.qx - r.qx) < 1e-10 and abs(o.qy - r.qy) < 1e-10
