@@ -160,7 +160,7 @@ def main():
         out["seed_adjoint"] = seed_adjoint_timing()
         _progress("batched adjoint done")
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(ring, flags, pool, args.turns, E, args.cpu_seconds)
+        out["cpu_baseline"] = cpu_baseline(ring, flags, pool, args.turns, E, args.cpu_seconds, gpu_lost=lost)
         _progress("CPU baseline done")
     print(json.dumps(out))
 
@@ -170,8 +170,10 @@ def fodo_track_timing(particles=4096, turns=1000, reps=3, cpu=True):
     (96 elements, KICK, 6D), 4096 particles (x, y ~ N(0, 1 mm), px, py ~
     N(0, 0.1 mrad), dp ~ N(0, 1e-3), seed 0), where = [-1], 1000 turns.
     GPU: kernel time (HIP events) and Track.run() wall time including the
-    196.6 MB copy of tracks back to the host; CPU: the oracle (gcc fast-math,
-    OpenMP) on the whole workload."""
+    196.6 MB copy of tracks back to the host, as run() does it (turns split
+    over launches, finished samples copied meanwhile) and in one launch
+    followed by one copy; CPU: the oracle (gcc fast-math, OpenMP) on the
+    whole workload."""
     import numpy as np
     import torch
     import ufo_amd as ufo
@@ -184,7 +186,8 @@ def fodo_track_timing(particles=4096, turns=1000, reps=3, cpu=True):
     tr.parameters[:] = bunch
     tr.run()
     stream = torch.cuda.current_stream()
-    k_ms, run_ms = [], []
+    k_ms, run_ms, one_ms = [], [], []
+    chunks = tr._chunks()
     for _ in range(reps):
         e = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
         torch.cuda.synchronize()
@@ -193,6 +196,11 @@ def fodo_track_timing(particles=4096, turns=1000, reps=3, cpu=True):
         e[1].record(stream)
         torch.cuda.synchronize()
         k_ms.append(e[0].elapsed_time(e[1]))
+        tr.copy_chunks = 1
+        t = time.perf_counter()
+        tr.run()
+        one_ms.append(1e3 * (time.perf_counter() - t))
+        tr.copy_chunks = None
         t = time.perf_counter()
         tr.run()
         run_ms.append(1e3 * (time.perf_counter() - t))
@@ -202,6 +210,7 @@ def fodo_track_timing(particles=4096, turns=1000, reps=3, cpu=True):
     E = tr.kernel.elements
     out = {"workload": f"16-cell FODO ({E} elements, KICK, 6D), {particles} particles, {turns} turns, where=[-1]",
            "gpu_kernel_ms": round(float(np.median(k_ms)), 3), "gpu_run_ms": round(float(np.median(run_ms)), 3),
+           "run_launches": chunks, "gpu_run_one_launch_ms": round(float(np.median(one_ms)), 3),
            "gpu_pet_per_s": pt * E / (float(np.median(k_ms)) * 1e-3), "survivors": int(alive.sum())}
     if cpu:
         from oracle import runner
@@ -405,9 +414,11 @@ def _host_cpu():
     return None
 
 
-def cpu_baseline(ring, flags, pool, turns, E, budget_s):
+def cpu_baseline(ring, flags, pool, turns, E, budget_s, gpu_lost=None):
     """The oracle (restated reference kernels, gcc fast-math build, OpenMP)
-    on an evenly strided sample of the grid, sized to ~budget_s seconds."""
+    on an evenly strided sample of the grid, sized to ~budget_s seconds.
+    With ``gpu_lost`` (the GPU's loss turns over the whole grid) it also
+    reports on how many sample points the two loss turns are identical."""
     import numpy as np
     from oracle import runner
     threads = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
@@ -425,10 +436,14 @@ def cpu_baseline(ring, flags, pool, turns, E, budget_s):
     lost = runner.stable_aperture(ring, sample, flags=flags, turns=turns, parameters=["x", "y"], mode="fast")
     dt = time.perf_counter() - t
     pt = int(np.minimum(lost.astype(np.int64) + 1, turns).sum())
-    return {"value": pt * E / dt, "unit": "particle-element-turns/s", "cores": threads, "kind": "port",
-            "host_cpu": _host_cpu(),
-            "sample": f"{len(sample)} grid points (every {stride}th of the {len(pool)}-point grid), "
-                      f"{turns} turns, {dt:.1f} s"}
+    out = {"value": pt * E / dt, "unit": "particle-element-turns/s", "cores": threads, "kind": "port",
+           "host_cpu": _host_cpu(),
+           "sample": f"{len(sample)} grid points (every {stride}th of the {len(pool)}-point grid), "
+                     f"{turns} turns, {dt:.1f} s"}
+    if gpu_lost is not None:
+        same = np.asarray(gpu_lost)[::stride][:m] == lost
+        out["lost_identical"] = f"{int(same.sum())}/{len(sample)}"
+    return out
 
 
 if __name__ == "__main__":

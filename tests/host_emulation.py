@@ -28,14 +28,15 @@ static ufo_dim3 blockIdx, threadIdx;
 """
 
 _DRIVER = r"""
-extern "C" void host_run(const ufloat *pool, ufloat *tracks, int particles, int block)
+extern "C" void host_run(const ufloat *pool, ufloat *tracks, ufloat *state, int particles, int block,
+                         int turn0, int turn1)
 {
     int blocks = (particles + block - 1) / block;
     for (int b = 0; b < blocks; ++b) {
         blockIdx.x = b;
         for (int t = 0; t < block; ++t) {
             threadIdx.x = t;
-            %s(pool, tracks, particles);
+            %s(pool, tracks, state, particles, turn0, turn1);
         }
     }
 }
@@ -60,13 +61,19 @@ def compile_track(kernel):
     return ctypes.CDLL(so)
 
 
-def run_track(kernel, pool, samples):
-    """Host run of a generated Track kernel: returns [particles, samples, 6]."""
+def run_track(kernel, pool, samples, splits=()):
+    """Host run of a generated Track kernel: returns [particles, samples, 6].
+    ``splits`` are the turns at which the run is cut into launches that
+    resume from the saved state, as Track.run does."""
     lib = compile_track(kernel)
     dt = np.float32 if kernel.fp32 else np.float64
     pool = np.ascontiguousarray(pool, dtype=dt)
     n = pool.shape[0]
     out = np.full((samples, 6, n), np.nan, dtype=dt)
-    lib.host_run(pool.ctypes.data_as(ctypes.c_void_p), out.ctypes.data_as(ctypes.c_void_p),
-                 ctypes.c_int(n), ctypes.c_int(kernel.block))
+    state = np.full((4, max(1, n)), np.nan, dtype=dt)
+    bounds = [0] + sorted(splits) + [kernel.turns]
+    for t0, t1 in zip(bounds[:-1], bounds[1:]):
+        lib.host_run(pool.ctypes.data_as(ctypes.c_void_p), out.ctypes.data_as(ctypes.c_void_p),
+                     state.ctypes.data_as(ctypes.c_void_p), ctypes.c_int(n), ctypes.c_int(kernel.block),
+                     ctypes.c_int(t0), ctypes.c_int(t1))
     return out.transpose(2, 0, 1)

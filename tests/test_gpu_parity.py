@@ -64,6 +64,33 @@ def test_fodo_track_6d():
     assert np.all(gpu[alive, :, 4] == 0.0)
 
 
+@pytest.mark.parametrize("double", [True, False])
+def test_track_run_split_over_launches(double):
+    """Track.run splits the turns over launches (copying finished samples
+    back meanwhile); one launch, 7 uneven pieces and one launch per turn give
+    the same tracks bit for bit, with particles lost part-way and a
+    fractional observation point on a thin element that is never written."""
+    ring = lattices.fodo_ring(16)
+    bunch = lattices.gaussian_bunch(300, seed=3)
+    bunch[:6, 0] = [0.035, 0.038, 0.045, 0.05, 0.0399, -0.039]
+    sf = ring.find(lambda e: isinstance(e, ufo.Multipole))[0]
+    flags = ufo.KICK | (ufo.DOUBLE_PRECISION if double else 0)
+    tr = ufo.Track(ring, flags=flags, turns=200, particles=len(bunch), parameters=COORD6,
+                   where=[2.5, sf + 0.5, -1])
+    tr.parameters[:] = bunch
+    runs = []
+    for chunks in (1, 7, 200):
+        tr.copy_chunks = chunks
+        tr.tracks[:] = 0.0
+        tr.run()
+        runs.append(np.array(tr.tracks))
+    lost = np.isnan(runs[0][:, 399, 0])
+    assert 0 < lost.sum() < len(bunch) and np.all(np.isnan(runs[0][:, 400:, :]))
+    u = np.uint64 if double else np.uint32
+    for r in runs[1:]:
+        np.testing.assert_array_equal(r.view(u), runs[0].view(u))
+
+
 def test_fodo_config0_full_size():
     """The reference's own CPU workload at full size: FODO-16, 4096
     particles (seed 0), 6D, 1000 turns, where=[-1].  After 1000 turns the

@@ -732,6 +732,7 @@ class Kernel:
     dumps_per_turn: int = 0
     block: int = 256
     period: tuple = (0, 0, 0)
+    turns: int = 0             # turns compiled in (Track: the last turn of a run)
 
 
 def _layout(body_ops, env, observe=None, loop_var="rep", emit=None):
@@ -961,7 +962,14 @@ def track_kernel(line, flags, turns, parameters, where, dp, block=256, name="ufo
     """Track: samples at the fracture points of ``where`` (and at the end of
     the line for -1), in lattice order, turn after turn; a particle stops
     after the turn at whose end x is NaN (ufo/track.py:62-114).  Output is
-    [sample][coordinate][particle]."""
+    [sample][coordinate][particle].
+
+    One launch runs turns [turn0, turn1).  A launch that starts after turn 0
+    resumes from ``state`` ([4][particles]: x, px, y, py; z and dp never
+    change and are loaded from the pool as at turn 0) and returns at once for
+    a particle already lost; a launch that ends before the last turn saves the
+    state.  Splitting the turns over launches changes no result, and lets the
+    caller copy finished samples back while later turns run."""
     where = list(where) if isinstance(where, (list, tuple)) else [where]
     pieces = line.method(fracture=where, flags=flags, parameters=parameters)
     ops = []
@@ -984,7 +992,8 @@ def track_kernel(line, flags, turns, parameters, where, dp, block=256, name="ufo
                            nan='__builtin_nanf("")' if env.fp32 else '__builtin_nan("")')
     code += f"""
 extern "C" __global__ void __launch_bounds__({block})
-{name}(const ufloat *__restrict__ pool, ufloat *__restrict__ tracks, int particles)
+{name}(const ufloat *__restrict__ pool, ufloat *__restrict__ tracks, ufloat *__restrict__ state,
+     int particles, int turn0, int turn1)
 {{
     {_decls(env)}
     const int idx = blockIdx.x * {block} + threadIdx.x;
@@ -992,15 +1001,23 @@ extern "C" __global__ void __launch_bounds__({block})
     const size_t np = (size_t)particles;
     const ufloat *__restrict__ row = pool + (size_t)idx * {P};
 {chr(10).join(load)}
-    int s = 0;
-    for (int turn = 0; turn < {turns}; ++turn) {{
+    if (turn0 > 0) {{
+        x = state[idx]; px = state[np + idx]; y = state[2 * np + idx]; py = state[3 * np + idx];
+        if (isnan(x)) return;
+    }}
+    int s = turn0 * {dumps};
+    for (int turn = turn0; turn < turn1; ++turn) {{
 {chr(10).join('        ' + s_ for s_ in body)}
         if (isnan(x)) break;
+    }}
+    if (turn1 < {turns}) {{
+        state[idx] = x; state[np + idx] = px; state[2 * np + idx] = y; state[3 * np + idx] = py;
     }}
 }}
 """
     return Kernel(name=name, src=code, kind="track", fp32=env.fp32, flops_per_turn=_count(ops, env),
-                  elements=element_count(line), dumps_per_turn=dumps, block=block, period=period)
+                  elements=element_count(line), dumps_per_turn=dumps, block=block, period=period,
+                  turns=int(turns))
 
 
 def optics_kernel(line, flags, where, parameters, name="ufo_optics") -> Kernel:

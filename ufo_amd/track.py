@@ -48,11 +48,19 @@ class Track(Simulation):
         self.tracks = self._tracks_host.permute(2, 0, 1).numpy()
         self._pool_dev = None
         self._tracks_dev = None
+        self._state_dev = None
+        self._copy_stream = None
+        self._events = []
+        # run() splits the turns over this many launches and copies each
+        # finished slab of samples back while the next launch runs; None
+        # picks 16 when the samples exceed 16 MB (else one launch).
+        self.copy_chunks = None
 
     def _buffers(self):
         if self._pool_dev is None:
             self._pool_dev = torch.empty(self._pool_host.shape, dtype=self.dtype, device=self.device)
             self._tracks_dev = torch.empty(self._tracks_host.shape, dtype=self.dtype, device=self.device)
+            self._state_dev = torch.empty((4, max(1, self.particles)), dtype=self.dtype, device=self.device)
 
     @property
     def tracks_device(self):
@@ -61,22 +69,61 @@ class Track(Simulation):
         self._buffers()
         return self._tracks_dev.permute(2, 0, 1)
 
+    def _launch_turns(self, turn0, turn1):
+        block = self.kernel.block
+        grid = (self.particles + block - 1) // block
+        self.module().launch(self.kernel.name, grid, block,
+                             [self._pool_dev, self._tracks_dev, self._state_dev, self.particles, turn0, turn1])
+
     def launch(self):
         """Enqueue one tracking pass on the current stream with the device
         pool as input (no host copies)."""
         self._buffers()
-        block = self.kernel.block
-        grid = (self.particles + block - 1) // block
         self._tracks_dev.fill_(float("nan"))
         if self.particles == 0:
             return
-        self.module().launch(self.kernel.name, grid, block,
-                             [self._pool_dev, self._tracks_dev, self.particles])
+        self._launch_turns(0, self.turns)
+
+    def _chunks(self):
+        n = self.copy_chunks
+        if n is None:
+            n = 16 if self._tracks_dev.numel() * self._tracks_dev.element_size() > (16 << 20) else 1
+        return max(1, min(int(n), self.turns))
 
     def run(self, threads=1):
-        """Copy ``parameters`` to the device, track, copy ``tracks`` back."""
+        """Copy ``parameters`` to the device, track, copy ``tracks`` back.
+        The turns are split over ``copy_chunks`` launches on the current
+        stream; a second stream copies each finished slab of samples to the
+        host while the next launch runs."""
         self._buffers()
+        stream = self.stream()
         self._pool_dev.copy_(self._host_pool(), non_blocking=True)
-        self.launch()
-        self._tracks_host.copy_(self._tracks_dev, non_blocking=True)
-        self.stream().synchronize()
+        self._tracks_dev.fill_(float("nan"))
+        n = self._chunks()
+        if self.particles == 0 or n == 1:
+            if self.particles:
+                self._launch_turns(0, self.turns)
+            self._tracks_host.copy_(self._tracks_dev, non_blocking=True)
+            stream.synchronize()
+            return
+        if self._copy_stream is None:
+            self._copy_stream = torch.cuda.Stream(self.device)
+            self._events = []
+        copier = self._copy_stream
+        while len(self._events) < n:
+            self._events.append(torch.cuda.Event())
+        bounds = [self.turns * k // n for k in range(n + 1)]
+        for k, (t0, t1) in enumerate(zip(bounds[:-1], bounds[1:])):
+            self._launch_turns(t0, t1)
+            done = self._events[k]
+            done.record(stream)
+            copier.wait_event(done)
+            # samples are numbered turn * dumps_per_turn + k; entries past
+            # turns * dumps_per_turn (points that are never written) go with
+            # the last slab
+            d = self.kernel.dumps_per_turn
+            a, b = t0 * d, (t1 * d if t1 < self.turns else self.samples)
+            with torch.cuda.stream(copier):
+                self._tracks_host[a:b].copy_(self._tracks_dev[a:b], non_blocking=True)
+        copier.synchronize()
+        stream.synchronize()
