@@ -40,11 +40,6 @@ FLAG_DOUBLE = 1 << 5
 FLAG_ACHROMATIC = 1 << 6
 
 
-def _s(v):
-    """Literal text of a value, as an f-string of the reference renders it."""
-    return f"{v}"
-
-
 # ---------------------------------------------------------------------------
 # Pass-method templates (ufo/methods.py)
 # ---------------------------------------------------------------------------
