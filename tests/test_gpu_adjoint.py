@@ -93,13 +93,14 @@ def test_gradient_fodo48_full_size():
     assert err.max() < 1e-6
 
 
-def test_initial_coordinate_adjoint_matches_tangent_jacobian():
+@pytest.mark.parametrize("cells,turns,n,every", [(8, 48, 256, 16), (48, 512, 65536, 32)])
+def test_initial_coordinate_adjoint_matches_tangent_jacobian(cells, turns, n, every):
     """dL/dx0 from the reverse pass equals J^T dL/dx_T with J from the
-    forward-tangent kernel of the same lattice (parameters baked in)."""
-    cells, turns = 8, 48
-    ring, shared, theta, bunch = _fodo_setup(cells, 256, seed=1)
+    forward-tangent kernel of the same lattice (parameters baked in); also
+    on the full adjoint workload (FODO-48, 65 536 particles, 512 turns)."""
+    ring, shared, theta, bunch = _fodo_setup(cells, n, seed=1)
     flags = ufo.KICK | ufo.DOUBLE_PRECISION
-    tr = ufo.AdjointTracker(ring, flags, turns, COORDS, shared, every=16)
+    tr = ufo.AdjointTracker(ring, flags, turns, COORDS, shared, every=every)
     dev = tr.device
     th = torch.tensor(theta, device=dev)
     x0 = torch.tensor(bunch, device=dev, requires_grad=True)
