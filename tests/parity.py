@@ -66,6 +66,33 @@ def reference_runs(track_fn, pool):
     return strict, fast, perturbed
 
 
+CHAOS_TOL = 1e-9
+
+
+def loss_turn_runs(sa_fn, pool, fp32=False, track_fn=None):
+    """Loss turns of the oracle (``sa_fn(pool, mode)``) and the particles
+    whose loss turn is regular: the strict run, the fast-math run and the
+    strict run from every input moved by one ulp (in the run's precision)
+    agree.  Over thousands of turns a particle at the chaotic edge of the
+    aperture can survive in all three runs by chance while its trajectories
+    have long parted; with ``track_fn(pool, mode)`` (the same particles
+    tracked turn by turn) a particle is regular only if, in addition, its
+    sensitivity over the run (fast-math and one-ulp-moved tracks against the
+    strict one) stays below CHAOS_TOL."""
+    strict = sa_fn(pool, "strict")
+    fast = sa_fn(pool, "fast")
+    dt = np.float32 if fp32 else np.float64
+    moved = np.nextafter(np.array(pool, dtype=dt), dt(np.inf))
+    perturbed = sa_fn(moved, "strict")
+    regular = (strict == fast) & (strict == perturbed)
+    if track_fn is not None:
+        t_strict = track_fn(pool, "strict")
+        sens = np.maximum(particle_err(track_fn(pool, "fast"), t_strict),
+                          particle_err(track_fn(moved, "strict"), t_strict))
+        regular &= sens <= CHAOS_TOL
+    return strict, regular
+
+
 def compare_tracks(gpu, strict, fast, perturbed):
     ns = nan_flags(strict)
     stable = np.all(ns == nan_flags(fast), axis=1) & np.all(ns == nan_flags(perturbed), axis=1)
