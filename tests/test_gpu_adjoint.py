@@ -186,3 +186,34 @@ def test_gradient_every_element_kind(exact):
         print(f"{p}: adjoint {g[i]:+.10e} fd {fd[i]:+.10e} rel {rel[i]:.2e}")
     assert 0 < int((lost < turns).sum()) < len(bunch) // 10
     assert np.all(rel < 1e-5)
+
+
+def test_argument_shapes_are_checked_before_launch():
+    """Inputs whose shapes do not match what the kernels index raise on the
+    host: x0 columns, the number of strengths, lam, backward before
+    forward (single-lattice and batched trackers)."""
+    from ufo_amd.batch_adjoint import BatchedAdjoint
+    ring, shared, theta, bunch = _fodo_setup(2, 64)
+    flags = ufo.KICK | ufo.DOUBLE_PRECISION
+    tr = ufo.AdjointTracker(ring, flags, 8, COORDS, shared, every=4)
+    dev = tr.device
+    with pytest.raises(RuntimeError):
+        tr.backward(torch.zeros((64, 4), dtype=torch.float64, device=dev))
+    with pytest.raises(ValueError):
+        tr.forward(torch.tensor(bunch[:, :3], device=dev), theta)
+    with pytest.raises(ValueError):
+        tr.forward(torch.tensor(bunch, device=dev), theta[:-1])
+    xf, lost = tr.forward(torch.tensor(bunch, device=dev), theta)
+    with pytest.raises(ValueError):
+        tr.backward(torch.zeros((63, 4), dtype=torch.float64, device=dev))
+    gx0, gth = tr.backward(torch.ones((64, 4), dtype=torch.float64, device=dev))
+    assert gth.shape == (len(theta),) and torch.isfinite(gth).all()
+
+    ring, prm, vals, counts, fam = lattices.dba_family_model(seeds=2, count_range=(10, 20))
+    with pytest.raises(ValueError):
+        BatchedAdjoint(ring, flags, 8, COORDS, prm, vals[:, :-1], counts, fam[:2])
+    ba = BatchedAdjoint(ring, flags, 8, COORDS, prm, vals, counts, fam[:2])
+    with pytest.raises(RuntimeError):
+        ba.backward(torch.zeros((ba.particles, 4), dtype=torch.float64, device=dev))
+    with pytest.raises(ValueError):
+        ba.forward(torch.zeros((ba.particles + 1, 5), dtype=torch.float64, device=dev))

@@ -242,3 +242,27 @@ def test_batched_config_full_size_sampled():
           f"{np.sum(strict < turns)}, regular {regular.sum()}, differ {np.sum(lost[pick] != strict)}")
     assert regular.sum() > 0.7 * len(pick)
     assert np.array_equal(lost[pick][regular], strict[regular])
+
+
+def test_batched_empty_seeds():
+    """Seeds with no particles between and around non-empty ones, and a
+    batch with no particles at all: the loss turns equal those of the plain
+    StableAperture kernel with each seed's errors as per-particle
+    parameters, bit for bit."""
+    turns = 200
+    ring, prm, vals, _ = lattices.dba_error_model(periods=4, seeds=6)
+    flags = ufo.KICK | ufo.DOUBLE_PRECISION | ufo.FIVED
+    counts = np.array([0, 150, 0, 0, 77, 0])
+    ba = BatchedAperture(ring, flags, turns, prm, vals, counts, block_turns=64)
+    pool = np.concatenate([lattices.seed_grid(int(c), xlim=0.03, ymax=0.015) for c in counts if c])
+    ba.parameters[:] = pool
+    ba.run()
+    seed_of = np.repeat(np.arange(len(counts)), counts)
+    sa = ufo.StableAperture(ring, flags=flags, turns=turns, particles=len(pool), parameters=["x", "y"] + prm)
+    sa.parameters[:] = np.concatenate([pool, vals[seed_of]], axis=1)
+    sa.run()
+    assert 0 < np.sum(sa.lost < turns) < len(pool)
+    assert np.array_equal(ba.lost, sa.lost)
+    empty = BatchedAperture(ring, flags, turns, prm, vals[:2], [0, 0], block_turns=64)
+    empty.run()
+    assert empty.lost.shape == (0,)

@@ -55,6 +55,19 @@ def test_batched_launch_shape_validation():
                    "{ ufloat t1 = x * y; }", "{ ufloat tb1 = xb * yb; }", "}"]
 
 
+def test_batched_counts_are_checked():
+    """One non-negative particle count per seed, at least one seed; seeds
+    with no particles are allowed (the device side needs none)."""
+    from ufo_amd.batch import BatchedAperture
+    ring, prm, vals, counts = lattices.dba_error_model(periods=4, seeds=3)
+    fl = ufo.KICK | ufo.DOUBLE_PRECISION | ufo.FIVED
+    for bad in ([], [5, -1, 2]):
+        with pytest.raises(ValueError):
+            BatchedAperture(ring, fl, 10, prm, vals[:len(bad)], bad, block_turns=64)
+    ba = BatchedAperture(ring, fl, 10, prm, vals, [0, 7, 0], block_turns=64)
+    assert ba.particles == 7 and list(ba.offsets) == [0, 0, 7, 7] and ba.parameters.shape == (7, 2)
+
+
 def test_fp32_host_evaluation_rounds_every_step():
     """ir.evaluate(fp32=True) reproduces an FP32 kernel's evaluation of the
     same expression: every leaf and operation rounded to float32."""

@@ -617,14 +617,19 @@ class AdjointTracker(Simulation):
         return ct
 
     def _theta(self, theta):
-        th = torch.as_tensor(theta, dtype=torch.float64, device=self.device).contiguous()
+        th = torch.as_tensor(theta, dtype=torch.float64, device=self.device).contiguous().reshape(-1)
+        if th.numel() != len(self.lattice_parameters):
+            raise ValueError(f"theta has {th.numel()} values for {len(self.lattice_parameters)} lattice parameters")
         if th.numel() == 0:
             th = torch.zeros(1, dtype=torch.float64, device=self.device)
         return th
 
     def forward(self, x0, theta):
+        """x0 [particles, len(coords)], theta [len(lattice_parameters)]."""
         fwd = self.modules()[0]
         x0 = torch.as_tensor(x0, dtype=torch.float64, device=self.device).contiguous()
+        if x0.dim() != 2 or x0.shape[1] != len(self.coords):
+            raise ValueError(f"x0 has shape {tuple(x0.shape)}; expected (particles, {len(self.coords)})")
         n = x0.shape[0]
         pool = x0 if x0.shape[1] > 0 else torch.zeros((n, 1), dtype=torch.float64, device=self.device)
         ct = self.coefficients(theta)
@@ -637,10 +642,16 @@ class AdjointTracker(Simulation):
         return xf, lost
 
     def backward(self, lam):
+        """lam = dL/d(final x, px, y, py) [particles, 4] for the last
+        ``forward``; returns (dL/dx0 [particles, 4], dL/dtheta)."""
         bwd = self.modules()[1]
+        if self._saved is None:
+            raise RuntimeError("backward() needs a forward() first")
         pool, th, ckpt, lost = self._saved
         n = pool.shape[0]
         lam = torch.nan_to_num(torch.as_tensor(lam, dtype=torch.float64, device=self.device), nan=0.0).contiguous()
+        if tuple(lam.shape) != (n, 4):
+            raise ValueError(f"lam has shape {tuple(lam.shape)}; expected ({n}, 4)")
         block = self.kbwd.block
         nb = max(1, (n + block - 1) // block)
         npar = max(1, len(self.lattice_parameters))

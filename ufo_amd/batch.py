@@ -297,8 +297,10 @@ class BatchedAperture(Simulation):
         self.turns = int(turns)
         self.coords = list(coords)
         self.lattice_parameters = list(lattice_parameters)
-        self.counts = np.asarray(counts, dtype=np.int64)
+        self.counts = np.asarray(counts, dtype=np.int64).reshape(-1)
         self.seeds = len(self.counts)
+        if self.seeds == 0 or np.any(self.counts < 0):
+            raise ValueError("counts: one non-negative particle count per seed, at least one seed")
         self.seed_values = np.asarray(seed_values, dtype=np.float64).reshape(self.seeds, -1)
         self.kernel, self.layout = batch_kernels(line, flags, self.coords, self.lattice_parameters, dp)
         self.table = build_table(self.layout, self.seed_values, self.fp32)

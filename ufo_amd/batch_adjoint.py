@@ -369,9 +369,14 @@ class BatchedAdjoint(Simulation):
         self.kfwd, self.kbwd, self.layout, self.ncheck = batched_adjoint_kernels(
             line, flags | DOUBLE_PRECISION, self.turns, self.coords, lattice_parameters, diff_parameters, dp, every)
         self.kernel = self.kfwd
-        self.seed_values = np.asarray(seed_values, dtype=np.float64)
+        self.counts = np.asarray(counts, dtype=np.int64).reshape(-1)
+        if len(self.counts) == 0 or np.any(self.counts < 0):
+            raise ValueError("counts: one non-negative particle count per seed, at least one seed")
+        self.seed_values = np.asarray(seed_values, dtype=np.float64).reshape(len(self.counts), -1)
+        if self.seed_values.shape[1] != len(lattice_parameters):
+            raise ValueError(f"seed_values has {self.seed_values.shape[1]} columns for "
+                             f"{len(lattice_parameters)} lattice parameters")
         self.table = batch.build_table(self.layout, self.seed_values)
-        self.counts = np.asarray(counts, dtype=np.int64)
         self.ndiff = max(1, len(diff_parameters))
         seeds, starts, cnts = [], [], []
         off = 0
@@ -396,25 +401,37 @@ class BatchedAdjoint(Simulation):
         return self._dev
 
     def forward(self, x0):
+        """x0 [particles, len(coords)], rows grouped by seed."""
         b = self._buffers()
         x0 = torch.as_tensor(x0, dtype=torch.float64, device=self.device).contiguous()
         n = self.particles
-        ckpt = torch.empty((self.ncheck, 4, n), dtype=torch.float64, device=self.device)
+        if tuple(x0.shape) != (n, len(self.coords)):
+            raise ValueError(f"x0 has shape {tuple(x0.shape)}; expected ({n}, {len(self.coords)})")
+        ckpt = torch.empty((self.ncheck, 4, max(1, n)), dtype=torch.float64, device=self.device)
         xf = torch.empty((n, 4), dtype=torch.float64, device=self.device)
         lost = torch.empty((n,), dtype=torch.int32, device=self.device)
         nt = len(self._tiles[0])
-        self._mods[0].launch(self.kfwd.name, nt, 64, [x0, b["table"], b["ts"], b["t0"], b["tc"], ckpt, xf, lost, n])
+        if nt:
+            self._mods[0].launch(self.kfwd.name, nt, 64,
+                                 [x0, b["table"], b["ts"], b["t0"], b["tc"], ckpt, xf, lost, n])
         self._saved = (x0, ckpt, lost)
         return xf, lost
 
     def backward(self, lam):
+        """lam = dL/d(final x, px, y, py) [particles, 4] (NaN rows of lost
+        particles count as 0) for the last ``forward``."""
         b = self._buffers()
+        if self._saved is None:
+            raise RuntimeError("backward() needs a forward() first")
         x0, ckpt, lost = self._saved
         n = self.particles
         lam = torch.nan_to_num(torch.as_tensor(lam, dtype=torch.float64, device=self.device), nan=0.0).contiguous()
+        if tuple(lam.shape) != (n, 4):
+            raise ValueError(f"lam has shape {tuple(lam.shape)}; expected ({n}, 4)")
         gx0 = torch.zeros((n, 4), dtype=torch.float64, device=self.device)
         gseed = torch.zeros((len(self.counts), self.ndiff), dtype=torch.float64, device=self.device)
         nt = len(self._tiles[0])
-        self._mods[1].launch(self.kbwd.name, nt, 64,
-                             [x0, b["table"], b["ts"], b["t0"], b["tc"], ckpt, lam, lost, gx0, gseed, n])
+        if nt:
+            self._mods[1].launch(self.kbwd.name, nt, 64,
+                                 [x0, b["table"], b["ts"], b["t0"], b["tc"], ckpt, lam, lost, gx0, gseed, n])
         return gx0, gseed
