@@ -143,7 +143,7 @@ def main():
             "traffic": measured_traffic(args),
             "kernel": sa.kernel.name,
             "kernel_ms": float(np.mean(kernel_ms)),
-            "peak_from_device": sa.ctx.device.fp64_peak() / 1e12,
+            "peak_from_device": sa.ctx.device.fp64_peak / 1e12,
         },
         "cpu_baseline": None,
     }
