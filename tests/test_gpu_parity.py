@@ -454,3 +454,26 @@ def test_reference_thick_vs_kick_case():
                 r = runner.Optics(alba.RING, flags=f)
                 r.run()
                 assert abs(o.qx - r.qx) < 1e-10 and abs(o.qy - r.qy) < 1e-10
+
+
+def test_nan_start_and_no_parameters():
+    """IEEE semantics on the device: a particle starting with x = NaN is lost
+    at turn 0 in StableAperture (fabs(NaN) > 1 is false, isnan catches it)
+    and its Track samples are NaN, as in the strict oracle; a Track with no
+    parameters runs every particle from the origin."""
+    ring = lattices.fodo_ring(4)
+    flags = ufo.KICK | ufo.DOUBLE_PRECISION
+    pool = lattices.gaussian_bunch(64, seed=2)
+    pool[[3, 17], 0] = np.nan
+    sa = ufo.StableAperture(ring, flags=flags, turns=50, particles=len(pool), parameters=COORD6)
+    sa.parameters[:] = pool
+    sa.run()
+    ref = runner.stable_aperture(ring, pool, flags=flags, turns=50, parameters=COORD6)
+    assert sa.lost[3] == 0 and sa.lost[17] == 0
+    assert np.array_equal(sa.lost, ref)
+    gpu, strict, fast, moved = _track_both(ring, pool, flags, 20, COORD6, [-1])
+    assert np.array_equal(np.isnan(gpu[:, :, 0]), np.isnan(strict[:, :, 0]))
+    assert np.all(np.isnan(gpu[[3, 17], :, 0]))
+    tr = ufo.Track(ring, flags=flags, turns=5, particles=8, parameters=[], where=[-1])
+    tr.run()
+    assert np.all(tr.tracks == 0.0)

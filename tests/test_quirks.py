@@ -1,0 +1,93 @@
+"""The reference's quirks that the drop-in keeps (or deliberately fixes),
+one test each, on the CPU: the element lowering and the emitted kernels
+are compared with the oracle, which restates the reference."""
+import numpy as np
+import pytest
+
+import ufo_amd as ufo
+from ufo_amd import codegen, ir
+from oracle import runner
+
+import host_emulation
+
+COORD6 = ["x", "px", "y", "py", "dp"]
+FL = ufo.KICK | ufo.DOUBLE_PRECISION
+
+
+def _bunch(n=32, seed=4):
+    rng = np.random.default_rng(seed)
+    return np.stack([rng.normal(0, 2e-3, n), rng.normal(0, 1e-4, n), rng.normal(0, 2e-3, n),
+                     rng.normal(0, 1e-4, n), rng.normal(0, 1e-3, n)], 1)
+
+
+def _ring(elem):
+    return ufo.Line("R", [ufo.Drift("D1", length=0.5), elem, ufo.Drift("D2", length=0.5),
+                          ufo.Quadrupole("QF", length=0.3, k1=1.1), ufo.Drift("D3", length=0.5),
+                          ufo.Quadrupole("QD", length=0.3, k1=-1.1)])
+
+
+@pytest.mark.parametrize("cls,main,skew", [(ufo.Sextupole, "k2", "k2s"), (ufo.Octupole, "k3", "k3s")])
+def test_skew_main_field_is_ignored(cls, main, skew):
+    """ufo/methods.py:219,227 pass the error list dks to teapot and drop
+    ksl, so k2s (sextupole) and k3s (octupole) have no effect: the lowering
+    with a skew strength equals the lowering without, and the oracle agrees
+    that the tracks are identical."""
+    a = cls("S", length=0.2, **{main: 30.0, skew: 7.0})
+    b = cls("S", length=0.2, **{main: 30.0, skew: 0.0})
+    assert a.method(flags=FL) == b.method(flags=FL)
+    pool = _bunch()
+    ta = runner.track(_ring(a), pool, flags=FL, turns=20, parameters=COORD6, where=[-1])
+    tb = runner.track(_ring(b), pool, flags=FL, turns=20, parameters=COORD6, where=[-1])
+    np.testing.assert_array_equal(ta, tb)
+
+
+def test_numeric_zero_override_falls_back_to_the_stored_value():
+    """Element.method resolves ``value or self.value`` (ufo/elements.py:201-207):
+    an explicit numeric 0 falls back to the stored strength, while a
+    parameter string binds."""
+    q = ufo.Quadrupole("Q", length=0.5, k1=1.2)
+    assert q.method(flags=FL, k1=0) == q.method(flags=FL)
+    bound = q.method(flags=FL, k1="parameters[0]")
+    assert bound != q.method(flags=FL)
+    assert any(e.params() == {0} for op in bound[0] for e in op.exprs())
+
+
+def test_mutable_defaults_are_not_shared_and_where_is_not_mutated():
+    """The reference's default lists are shared between instances and
+    Optics appends -1 to the caller's ``where`` (ufo/optics.py:55-56); here
+    every element owns its lists and the caller's list is left alone."""
+    a, b = ufo.Quadrupole("A"), ufo.Quadrupole("B")
+    a.dkn.append(1.0)
+    assert b.dkn == []
+    line = _ring(ufo.Marker("M"))
+    where = [2]
+    o = ufo.Optics(line, where=where, flags=ufo.LINEAR | ufo.ACHROMATIC | ufo.DOUBLE_PRECISION)
+    assert where == [2] and o.count == 2
+
+
+def test_track_without_parameters_is_accepted():
+    """ufo/track.py:57 sizes the pool [particles, len(parameters)], so
+    parameters=[] fails there; here the pool has one unused column and every
+    particle starts at the origin (which a linear ring keeps)."""
+    line = _ring(ufo.Marker("M"))
+    k = codegen.track_kernel(line, FL, 5, [], [-1], 0.0)
+    host = host_emulation.run_track(k, np.zeros((4, 1)), 5)
+    assert np.all(host == 0.0)
+    tr = ufo.Track(line, flags=FL, turns=5, particles=4, parameters=[], where=[-1])
+    assert tr.parameters.shape == (4, 1)
+
+
+def test_nan_start_is_lost_at_turn_zero():
+    """The kernels keep IEEE semantics, so isnan(x) is never folded away
+    (under -cl-fast-relaxed-math the reference's compiler may assume no
+    NaN): a particle whose x is NaN stops after its first turn, exactly as
+    in the strict oracle."""
+    line = _ring(ufo.Marker("M"))
+    pool = _bunch(4)
+    pool[1, 0] = np.nan
+    k = codegen.track_kernel(line, FL, 6, COORD6, [-1], 0.0)
+    assert "isnan(x)" in k.src
+    host = host_emulation.run_track(k, pool, 6)
+    ref = runner.track(line, pool, flags=FL, turns=6, parameters=COORD6, where=[-1])
+    np.testing.assert_array_equal(np.isnan(host[:, :, 0]), np.isnan(ref[:, :, 0]))
+    assert np.all(np.isnan(host[1, :, 0])) and not np.any(np.isnan(host[0, :, 0]))
