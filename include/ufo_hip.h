@@ -90,6 +90,9 @@ int ufo_module_compile(ufo_ctx *ctx, const char *src, const char *options, const
 
 /* Load a code object produced by ufo_compile. */
 int ufo_module_load(ufo_ctx *ctx, const void *code, size_t size, ufo_module **mod);
+/* Unload a module.  The caller makes sure no kernel of it is still queued or
+ * running (synchronise its streams first); the Python drivers keep modules
+ * for the life of their context. */
 int ufo_module_destroy(ufo_module *mod);
 
 /* Register usage of a kernel: arch VGPRs, SGPRs, private bytes per lane,
@@ -122,6 +125,8 @@ int ufo_memcpy_dtoh(void *dst, const void *src, size_t bytes);
 int ufo_graph_begin(void *stream);
 int ufo_graph_end(void *stream, ufo_graph **graph);
 int ufo_graph_launch(ufo_graph *graph, void *stream);
+/* Free a graph.  The caller makes sure no replay of it is still queued or
+ * running (synchronise the streams it was launched on first). */
 int ufo_graph_destroy(ufo_graph *graph);
 
 /* Elapsed milliseconds of `kernel` launched `reps` times back to back on

@@ -434,7 +434,8 @@ class BatchedAperture(Simulation):
 
     def launch(self):
         """Enqueue the whole run (the captured graph) on this object's stream,
-        from the device pool."""
+        from the device pool.  The graphs are freed with the object: keep it
+        alive until that stream has been synchronised."""
         if self.block_turns is None or self.waves_per_simd is None:
             if self.block_turns is None:
                 self.tune()
