@@ -55,13 +55,14 @@ def nan_flags(t):
     return np.isnan(np.asarray(t, dtype=np.float64)[:, :, 0])
 
 
-def reference_runs(track_fn, pool):
+def reference_runs(track_fn, pool, fp32=False):
     """strict, fast and one-ulp-perturbed strict runs of the oracle;
-    ``track_fn(pool, mode)`` runs it."""
+    ``track_fn(pool, mode)`` runs it.  The ulp is that of the run's
+    precision (an FP64 ulp vanishes when an FP32 run rounds its inputs)."""
     strict = track_fn(pool, "strict")
     fast = track_fn(pool, "fast")
-    moved = np.array(pool, dtype=np.float64, copy=True)
-    moved = np.nextafter(moved, np.inf)
+    dt = np.float32 if fp32 else np.float64
+    moved = np.nextafter(np.array(pool, dtype=dt), dt(np.inf))
     perturbed = track_fn(moved, "strict")
     return strict, fast, perturbed
 

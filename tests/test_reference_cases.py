@@ -48,7 +48,7 @@ def test_elements_lattice_method_and_emitted_kernel(flags):
     def oracle(p, mode):
         return runner.track(line, p, flags=flags, turns=5, parameters=COORD6, where=[1, 3.5, -1], mode=mode)
 
-    strict, fast, moved = reference_runs(oracle, pool)
+    strict, fast, moved = reference_runs(oracle, pool, fp32=not (flags & ufo.DOUBLE_PRECISION))
     r = compare_tracks(host, strict, fast, moved)
     print(f"flags {flags}: " + summary(r))
     if flags & ufo.DOUBLE_PRECISION:
