@@ -39,3 +39,33 @@ def test_no_device_errors_and_compile_without_device():
     assert lib.ufo_launch(None, b"k", None, None, 0, None) == -1
     code = runtime.compile_only('extern "C" __global__ void k(double *a) { a[threadIdx.x] += 1.0; }')
     assert code[:4] in (b"\x7fELF", b"__CL")
+
+
+def test_product_fails_loudly_without_the_library(monkeypatch, tmp_path):
+    """No CPU fallback: with libufo_hip.so absent, loading the device layer
+    raises (and so does every simulation's run(), which goes through it)."""
+    from ufo_amd import runtime
+    monkeypatch.setattr(runtime, "_lib", None)
+    monkeypatch.setattr(runtime, "LIB", str(tmp_path / "libufo_hip.so"))
+    with pytest.raises(runtime.UfoError, match="missing"):
+        runtime.lib()
+    with pytest.raises(runtime.UfoError):
+        runtime.device_count()
+
+
+def test_run_without_a_device_raises():
+    """On a machine with no HIP device the drivers raise instead of
+    computing anything on the CPU."""
+    import torch
+    import ufo_amd as ufo
+    from ufo_amd import lattices, runtime
+    if torch.cuda.is_available():
+        pytest.skip("a HIP device is present")
+    tr = ufo.Track(lattices.fodo_ring(2), flags=ufo.KICK | ufo.DOUBLE_PRECISION, turns=2, particles=4,
+                   parameters=["x"], where=[-1])
+    with pytest.raises(runtime.UfoError, match="no HIP device"):
+        tr.run()
+    sa = ufo.StableAperture(lattices.fodo_ring(2), flags=ufo.KICK | ufo.DOUBLE_PRECISION, turns=2, particles=4,
+                            parameters=["x"])
+    with pytest.raises(runtime.UfoError, match="no HIP device"):
+        sa.run()
