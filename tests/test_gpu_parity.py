@@ -496,3 +496,19 @@ def test_nan_start_and_no_parameters():
     tr = ufo.Track(ring, flags=flags, turns=5, particles=8, parameters=[], where=[-1])
     tr.run()
     assert np.all(tr.tracks == 0.0)
+
+
+@pytest.mark.parametrize("flags", [ufo.KICK | ufo.DOUBLE_PRECISION, ufo.KICK | ufo.DOUBLE_PRECISION | ufo.FIVED])
+def test_observation_point_in_every_cell(flags):
+    """One observation point inside each cell's first quadrupole of the
+    16-cell ring: the stores sit in the period loop; samples in lattice
+    order, parity with the oracle (and, FIVED, the host build bit for bit)."""
+    ring = lattices.fodo_ring(16)
+    cell = len(ring.flatten()) // 16
+    where = [c * cell + 0.5 for c in range(16)] + [-1]
+    bunch = lattices.gaussian_bunch(256, seed=8)
+    gpu, strict, fast, moved = _track_both(ring, bunch, flags, 50, COORD6, where)
+    assert gpu.shape == (256, 50 * 17, 6)
+    r = compare_tracks(gpu, strict, fast, moved)
+    print(summary(r))
+    assert_parity(r)

@@ -76,3 +76,19 @@ def test_track_split_over_launches_is_bit_identical(splits):
     lost = np.isnan(one[:, 119, 0])              # the last written sample
     assert 0 < lost.sum() < len(bunch) and np.all(np.isnan(one[:, 120:, :]))
     np.testing.assert_array_equal(split.view(np.uint64), one.view(np.uint64))
+
+
+def test_observation_point_in_every_cell_of_a_periodic_ring():
+    """An observation point inside the first quadrupole of each of the 16
+    cells: the repeated cell (with its store) becomes the period loop, and
+    the samples are numbered in lattice order as in the reference."""
+    ring = lattices.fodo_ring(16)
+    cell = len(ring.flatten()) // 16
+    where = [c * cell + 0.5 for c in range(16)] + [-1]
+    k = codegen.track_kernel(ring, ufo.KICK | ufo.DOUBLE_PRECISION, 8, COORD6, where, 0.0)
+    assert k.dumps_per_turn == 17 and k.period[2] >= 2 and "for (int rep" in k.src
+    bunch = lattices.gaussian_bunch(64, seed=8)
+    host, strict, fast, moved = _host_vs_oracle(ring, bunch, ufo.KICK | ufo.DOUBLE_PRECISION, 8, COORD6, where)
+    r = compare_tracks(host, strict, fast, moved)
+    print(summary(r))
+    assert_parity(r)
