@@ -256,9 +256,22 @@ def _thick_pieces(fracture, make):
     return code
 
 
+KINDS = ("Marker", "Drift", "Multipole", "Quadrupole", "Sbend", "Rbend", "Sextupole", "Octupole", "Wire",
+         "Aperture")
+
+
 def element_text(el, flags, fracture, kw):
-    """Code pieces of one element, ufo/elements.py:56-518."""
-    kind = type(el).__name__
+    """Code pieces of one element, ufo/elements.py:56-518.
+
+    A user subclass of an element class whose ``method()`` calls the base
+    method with overrides (the HOWTO's time-dependent kicker,
+    HOWTO.md:86-98) is restated by the base class's template with those
+    overrides, which the subclass lists as ``oracle_overrides`` (name ->
+    the C text its method passes)."""
+    kind = next((c.__name__ for c in type(el).__mro__ if c.__name__ in KINDS), type(el).__name__)
+    extra = getattr(el, "oracle_overrides", None)
+    if extra:
+        kw = {**kw, **extra}
     g = lambda name: kw.get(name) or getattr(el, name)   # noqa: E731  (v or self.v)
     if kind == "Marker":
         return [""]

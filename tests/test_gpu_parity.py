@@ -512,3 +512,39 @@ def test_observation_point_in_every_cell(flags):
     r = compare_tracks(gpu, strict, fast, moved)
     print(summary(r))
     assert_parity(r)
+
+
+@pytest.mark.parametrize("double", [True, False])
+def test_howto_time_dependent_kicker(double):
+    """HOWTO.md:86-120 on the GPU: a user subclass of Multipole whose kick
+    reads the turn counter, inserted at the start of ALBA (thick, FIVED),
+    100 particles of the equilibrium emittance, 65 turns: the emittance
+    jumps after turn 20, and the tracks agree with the oracle (FP64: the
+    parity bar; FP32, the HOWTO's own precision: the statistical bar)."""
+    from test_reference_cases import nlk_ring, nlk_bunch
+    ring = nlk_ring()
+    pool = nlk_bunch()
+    flags = ufo.FIVED | (ufo.DOUBLE_PRECISION if double else 0)
+    gpu, strict, fast, moved = _track_both(ring, pool, flags, 65, ["x", "px"], [-1])
+    r = compare_tracks(gpu, strict, fast, moved)
+    print(summary(r))
+    assert_parity(r, min_regular_frac=0.0)
+    ex = np.mean(gpu[:, :, 0] ** 2 / 11.26 + gpu[:, :, 1] ** 2 * 11.26, axis=0)
+    assert abs(ex[19] - ex[0]) < 0.15 * ex[0] and ex[25] > 1.1 * ex[19]
+
+
+def test_alba_thick_small_amplitude_fp64():
+    """Thick ALBA (FIVED, FP64) with a bunch of the equilibrium emittance,
+    200 turns: nearly linear motion, where the reference's own rounding
+    noise is smallest and a one-ulp bias in a folded matrix coefficient
+    would dominate (the constants are folded correctly rounded)."""
+    alba = ufo.Lattice(os.path.join(OPTICS_DIR, "alba.mad"))
+    rng = np.random.default_rng(5)
+    n = 128
+    pool = np.stack([rng.normal(0, 1.5e-4, n), rng.normal(0, 1.5e-5, n),
+                     rng.normal(0, 5e-5, n), rng.normal(0, 5e-6, n)], 1)
+    flags = ufo.FIVED | ufo.DOUBLE_PRECISION
+    gpu, strict, fast, moved = _track_both(alba.RING, pool, flags, 200, ["x", "px", "y", "py"], [-1])
+    r = compare_tracks(gpu, strict, fast, moved)
+    print(summary(r))
+    assert_parity(r, min_regular_frac=0.0)

@@ -18,6 +18,7 @@ import re
 from dataclasses import dataclass, field, fields
 from typing import Tuple
 
+import mpmath
 import numpy as np
 
 from .flags import LINEAR, EXACT, ACHROMATIC
@@ -203,7 +204,7 @@ def evaluate(e: Expr, params, env=None, fp32=False) -> float:
     if k == "neg":
         return -evaluate(t[1], params, env, fp32)
     if k == "f":
-        return r32(_LIBM[t[1]](evaluate(t[2], params, env, fp32)))
+        return libm(t[1], evaluate(t[2], params, env, fp32), fp32)
     if k == "?":
         return evaluate(t[2] if evaluate(t[1], params, env, fp32) != 0.0 else t[3], params, env, fp32)
     a = evaluate(t[1], params, env, fp32)
@@ -257,10 +258,28 @@ DP = var("dp")
 ZERO = const(0.0)
 
 
-_LIBM = {
-    "sqrt": math.sqrt, "fabs": math.fabs, "sin": math.sin, "cos": math.cos,
-    "tan": math.tan, "sinh": math.sinh, "cosh": math.cosh,
-}
+_MP = {"sqrt": mpmath.sqrt, "sin": mpmath.sin, "cos": mpmath.cos, "tan": mpmath.tan,
+       "sinh": mpmath.sinh, "cosh": mpmath.cosh}
+_LIBM = {"sqrt": math.sqrt, "sin": math.sin, "cos": math.cos, "tan": math.tan,
+         "sinh": math.sinh, "cosh": math.cosh}
+
+
+def libm(name: str, v: float, fp32: bool = False) -> float:
+    """``name(v)`` correctly rounded to the kernel precision, as a C
+    compiler's constant folder computes it (gcc folds through MPFR): the
+    host's libm may be an ulp off, which over many turns of a thick
+    element would bias the orbit against a reference that folds exactly."""
+    if name == "fabs":
+        return abs(v)
+    if math.isnan(v) or math.isinf(v) or (name == "sqrt" and v < 0):
+        r = _LIBM[name](v) if not (name == "sqrt" and v < 0) else math.nan
+        return float(np.float32(r)) if fp32 else r
+    with mpmath.workprec(256):
+        r = _MP[name](mpmath.mpf(v))
+    if fp32:
+        with mpmath.workprec(24):
+            r = +r
+    return float(r)
 
 
 def fold(e: Expr, fp32: bool = False, env=None) -> Expr:
@@ -297,9 +316,7 @@ def fold(e: Expr, fp32: bool = False, env=None) -> Expr:
     if k == "f":
         a = fold(t[2], fp32, env)
         if a.is_const():
-            if fp32:
-                return const(float(np.float32(_LIBM[t[1]](float(np.float32(a.value))))))
-            return const(_LIBM[t[1]](a.value))
+            return const(libm(t[1], float(np.float32(a.value)) if fp32 else a.value, fp32))
         return Expr(("f", t[1], a))
     a = fold(t[1], fp32, env)
     b = fold(t[2], fp32, env)
