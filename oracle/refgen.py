@@ -362,10 +362,27 @@ def line_text(line, fracture=(), flags=0, parameters=()):
 # Kernel bodies as C11 functions
 # ---------------------------------------------------------------------------
 
+# The reference calls its platform's sin, cos, tan, sinh, cosh (OpenCL
+# builtins, whose rounding the standard leaves open).  The restatement
+# evaluates them in long double and rounds once to the kernel precision:
+# correctly rounded but for ~3e-4 of arguments (those within ~2^-11 ulp of a
+# rounding boundary; checked against mpmath), where the host's double sinh and
+# cosh are an ulp off for about a quarter of all arguments.  A gcc constant
+# fold of the same call (a literal dp) is correctly rounded through MPFR.
 _PRELUDE = """#include <tgmath.h>
 #include <math.h>
 #include <stddef.h>
 typedef {ufloat} ufloat;
+#undef sin
+#undef cos
+#undef tan
+#undef sinh
+#undef cosh
+#define sin(v) ((ufloat)sinl((long double)(v)))
+#define cos(v) ((ufloat)cosl((long double)(v)))
+#define tan(v) ((ufloat)tanl((long double)(v)))
+#define sinh(v) ((ufloat)sinhl((long double)(v)))
+#define cosh(v) ((ufloat)coshl((long double)(v)))
 """
 
 

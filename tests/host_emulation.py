@@ -20,10 +20,11 @@ _PRELUDE = r"""
 #include <cmath>
 #include <cstddef>
 using std::fma; using std::sqrt; using std::fabs; using std::sin; using std::cos; using std::tan;
-using std::sinh; using std::cosh; using std::isnan;
+using std::sinh; using std::cosh; using std::isnan; using std::rint; using std::fmod; using std::ldexp;
 struct ufo_dim3 { unsigned x, y, z; };
 static ufo_dim3 blockIdx, threadIdx;
 #define __global__
+#define __device__
 #define __launch_bounds__(n)
 """
 

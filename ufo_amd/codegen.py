@@ -36,7 +36,7 @@ from typing import Dict, List, Optional
 
 import numpy as np
 
-from . import ir
+from . import crmath, ir
 from .flags import FIVED, DOUBLE_PRECISION
 from .ir import Expr, fold, is_zero
 
@@ -158,7 +158,10 @@ class Env:
         if name is None:
             # children first, so every shared sub-expression is computed once
             if k == "f":
-                text = f"{e.t[1]}({self.val(e.t[2])})"
+                if e.t[1] in crmath.FUNCS:      # once per particle: correctly rounded
+                    text = crmath.call(e.t[1], self.val(e.t[2]), self.fp32)
+                else:
+                    text = f"{e.t[1]}({self.val(e.t[2])})"
             elif k == "?":
                 text = f"({self.val(e.t[1])} ? {self.val(e.t[2])} : {self.val(e.t[3])})"
             else:
@@ -227,6 +230,16 @@ class Env:
     def new_tmp(self) -> str:
         self.tmp += 1
         return f"t{self.tmp}"
+
+    def text_invariant(self, key: str, text: str) -> str:
+        """A per-particle invariant given as C text (keyed by ``key``)."""
+        k = Expr(("r", f"@inv:{key}"))
+        name = self.invariants.get(k)
+        if name is None:
+            name = f"I{len(self.invariants)}"
+            self.invariants[k] = name
+            self.inv_code.append(f"{name} = {text};")
+        return name
 
 
 # ---------------------------------------------------------------------------
@@ -502,11 +515,55 @@ def split_exact(e: Expr, env: Env):
     return f, None
 
 
+def split_runtime(e: Expr, env: Env) -> Optional[str]:
+    """Low part of a per-particle coefficient hoisted as an invariant whose
+    last operation is a product or quotient of two invariants: the exact
+    product error fma(a, b, -ab), or the quotient residual
+    fma(-q, b, a) / b, so that hi + lo is the exact (to 2^-106) value, as
+    ``split_exact`` gives for constants.  None where there is no such part
+    (or in FP32, table or in-loop contexts, where coefficients are not
+    invariants)."""
+    if env.fp32 or env.table is not None or env.param_mem or (env.stride is not None and env.stride[1] != 0):
+        return None
+    f = env.f(e)
+    if f.is_const() or f.has_raw() or f.uses_var("turn"):
+        return None
+    t = f.t
+    if t[0] == "neg":
+        lo = split_runtime(t[1], env)
+        return None if lo is None else env.text_invariant(f"neg {lo}", f"(-{lo})")
+    if t[0] == "?":
+        la, lb = split_runtime(t[2], env), split_runtime(t[3], env)
+        if la is None and lb is None:
+            return None
+        c = env.val(t[1])
+        zero = env.lit(0.0)
+        return env.text_invariant(f"sel {c} {la} {lb}", f"({c} ? {la or zero} : {lb or zero})")
+    if t[0] not in ("*", "/"):
+        return None
+    hi = env.val(f)
+    if not hi.startswith("I"):
+        return None
+    a, b = env.val(t[1]), env.val(t[2])
+    if t[0] == "*":
+        return env.text_invariant(f"lo* {hi}", f"fma({a}, {b}, -{hi})")
+    return env.text_invariant(f"lo/ {hi}", f"(fma(-{hi}, {b}, {a}) / {b})")
+
+
 def _thick_plane(env: Env, u: str, pu: str, m11, m12, m21, m22, du=None, dpu=None):
-    """u' = m11 u + m12 pu (+ du); pu' = m21 u + m22 pu (+ dpu); constant
-    coefficients carry their exact low parts."""
+    """u' = m11 u + m12 pu (+ du); pu' = m21 u + m22 pu (+ dpu); coefficients
+    carry their exact low parts (constants: ``split_exact``; per-particle
+    invariants: ``split_runtime``)."""
     a = env.new_tmp()
-    (h11, l11), (h12, l12), (h21, l21), (h22, l22) = (split_exact(m, env) for m in (m11, m12, m21, m22))
+    pairs = []
+    for m in (m11, m12, m21, m22):
+        hi, lo = split_exact(m, env)
+        if lo is None:
+            name = split_runtime(m, env)
+            if name is not None:
+                lo = Expr(("r", name))
+        pairs.append((hi, lo))
+    (h11, l11), (h12, l12), (h21, l21), (h22, l22) = pairs
 
     def row(hu, lu, hp, lp, add):
         acc = None if add is None else env.val(add)
@@ -733,6 +790,9 @@ class Kernel:
     block: int = 256
     period: tuple = (0, 0, 0)
     turns: int = 0             # turns compiled in (Track: the last turn of a run)
+
+    def __post_init__(self):
+        self.src = crmath.attach(self.src)
 
 
 def _layout(body_ops, env, observe=None, loop_var="rep", emit=None):
