@@ -197,14 +197,16 @@ def fodo_track_timing(particles=4096, turns=1000, reps=3, cpu=True):
         e[1].record(stream)
         torch.cuda.synchronize()
         k_ms.append(e[0].elapsed_time(e[1]))
-        tr.copy_chunks = 1
-        t = time.perf_counter()
+    # each way of running, repeated as a caller would: run() as it is, then
+    # one launch followed by one copy
+    for mode, times in ((None, run_ms), (1, one_ms)):
+        tr.copy_chunks = mode
         tr.run()
-        one_ms.append(1e3 * (time.perf_counter() - t))
-        tr.copy_chunks = None
-        t = time.perf_counter()
-        tr.run()
-        run_ms.append(1e3 * (time.perf_counter() - t))
+        for _ in range(reps):
+            t = time.perf_counter()
+            tr.run()
+            times.append(1e3 * (time.perf_counter() - t))
+    tr.copy_chunks = None
     alive = np.isfinite(tr.tracks[:, -1, 0])
     nan_turn = np.argmax(~np.isfinite(tr.tracks[:, :, 0]), axis=1)
     pt = int(np.where(alive, turns, nan_turn + 1).sum())

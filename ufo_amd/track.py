@@ -49,6 +49,7 @@ class Track(Simulation):
         self._pool_dev = None
         self._tracks_dev = None
         self._state_dev = None
+        self._run_stream = None
         self._copy_stream = None
         self._events = []
         # run() splits the turns over this many launches and copies each
@@ -92,38 +93,46 @@ class Track(Simulation):
 
     def run(self, threads=1):
         """Copy ``parameters`` to the device, track, copy ``tracks`` back.
-        The turns are split over ``copy_chunks`` launches on the current
-        stream; a second stream copies each finished slab of samples to the
-        host while the next launch runs."""
+        Split runs (``copy_chunks`` launches) go to a stream of this object
+        and a second one copies each finished slab of samples to the host
+        while the next launch runs.  The two streams are created one after
+        the other, which puts them on different hardware queues (a copy on
+        the queue of the launches would wait for them)."""
         self._buffers()
-        stream = self.stream()
-        self._pool_dev.copy_(self._host_pool(), non_blocking=True)
-        self._tracks_dev.fill_(float("nan"))
+        caller = self.stream()
         n = self._chunks()
         if self.particles == 0 or n == 1:
+            self._pool_dev.copy_(self._host_pool(), non_blocking=True)
+            self._tracks_dev.fill_(float("nan"))
             if self.particles:
                 self._launch_turns(0, self.turns)
             self._tracks_host.copy_(self._tracks_dev, non_blocking=True)
-            stream.synchronize()
+            caller.synchronize()
             return
         if self._copy_stream is None:
+            self._run_stream = torch.cuda.Stream(self.device)
             self._copy_stream = torch.cuda.Stream(self.device)
             self._events = []
-        copier = self._copy_stream
-        while len(self._events) < n:
+        runner, copier = self._run_stream, self._copy_stream
+        while len(self._events) < n + 1:
             self._events.append(torch.cuda.Event())
+        self._events[n].record(caller)           # work the caller queued on these buffers
+        runner.wait_event(self._events[n])
         bounds = [self.turns * k // n for k in range(n + 1)]
-        for k, (t0, t1) in enumerate(zip(bounds[:-1], bounds[1:])):
-            self._launch_turns(t0, t1)
-            done = self._events[k]
-            done.record(stream)
-            copier.wait_event(done)
-            # samples are numbered turn * dumps_per_turn + k; entries past
-            # turns * dumps_per_turn (points that are never written) go with
-            # the last slab
-            d = self.kernel.dumps_per_turn
-            a, b = t0 * d, (t1 * d if t1 < self.turns else self.samples)
-            with torch.cuda.stream(copier):
-                self._tracks_host[a:b].copy_(self._tracks_dev[a:b], non_blocking=True)
+        d = self.kernel.dumps_per_turn
+        with torch.cuda.stream(runner):
+            self._pool_dev.copy_(self._host_pool(), non_blocking=True)
+            self._tracks_dev.fill_(float("nan"))
+            for k, (t0, t1) in enumerate(zip(bounds[:-1], bounds[1:])):
+                self._launch_turns(t0, t1)
+                done = self._events[k]
+                done.record(runner)
+                copier.wait_event(done)
+                # samples are numbered turn * dumps_per_turn + k; entries past
+                # turns * dumps_per_turn (points that are never written) go
+                # with the last slab
+                a, b = t0 * d, (t1 * d if t1 < self.turns else self.samples)
+                with torch.cuda.stream(copier):
+                    self._tracks_host[a:b].copy_(self._tracks_dev[a:b], non_blocking=True)
         copier.synchronize()
-        stream.synchronize()
+        runner.synchronize()
