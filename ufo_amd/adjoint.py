@@ -321,6 +321,7 @@ def emit_kick_reverse_factored(op: ir.Kick, env: Env, out: List[str], g: _Grad):
 
 def emit_thick_reverse(op, env: Env, out: List[str], g: _Grad):
     """x_post = M x_pre + d per plane (det M = 1): x_pre = M^-1 (x_post - d),
+    the strengths' gradient lambda_post^T (dM x_pre + dd), then
     lambda_pre = M^T lambda_post."""
     if isinstance(op, ir.ThickQuad):
         mx, my = codegen.thick_quad_planes(op)
@@ -329,17 +330,23 @@ def emit_thick_reverse(op, env: Env, out: List[str], g: _Grad):
         mx, my = codegen.thick_bend_planes(op)
         dx = None if is_zero(env.f(mx[4])) else mx[4]
         dpx = None if is_zero(env.f(mx[5])) else mx[5]
-    for e in list(mx) + list(my):
-        if any(i >= env.shared_offset for i in ir.Expr.of(e).params()):
-            raise NotImplementedError("gradient with respect to thick-element parameters")
     for (u, pu, lu, lpu), m, du, dpu in ((("x", "px", "lx", "lpx"), mx, dx, dpx),
                                          (("y", "py", "ly", "lpy"), my, None, None)):
         v11, v12, v21, v22 = (env.val(e) for e in m[:4])
         uu = f"({u} - {env.val(du)})" if du is not None else u
         pp = f"({pu} - {env.val(dpu)})" if dpu is not None else pu
         out.append(f"{{ ufloat a = {uu}, b = {pp}; {u} = fma({v22}, a, -({v12}) * b); "
-                   f"{pu} = fma({v11}, b, -({v21}) * a); "
-                   f"ufloat la = {lu}; {lu} = fma({v11}, la, {v21} * {lpu}); "
+                   f"{pu} = fma({v11}, b, -({v21}) * a); }}")
+        # dL/dtheta += lambda_post^T (dM/dtheta x_pre + dd/dtheta)
+        _add_grad(env, g, m[0], [(lu, u)], out)
+        _add_grad(env, g, m[1], [(lu, pu)], out)
+        _add_grad(env, g, m[2], [(lpu, u)], out)
+        _add_grad(env, g, m[3], [(lpu, pu)], out)
+        if du is not None:
+            _add_grad(env, g, du, lu, out)
+        if dpu is not None:
+            _add_grad(env, g, dpu, lpu, out)
+        out.append(f"{{ ufloat la = {lu}; {lu} = fma({v11}, la, {v21} * {lpu}); "
                    f"{lpu} = fma({v12}, la, {v22} * {lpu}); }}")
 
 
