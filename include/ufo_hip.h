@@ -101,8 +101,9 @@ int ufo_kernel_info(ufo_module *mod, const char *kernel, int *vgprs, int *sgprs,
                     int *private_bytes, int *max_threads);
 
 /* Launch `kernel` with `nargs` arguments (args[i] points at the i-th
- * argument value) on `stream` (a hipStream_t, NULL = default stream);
- * asynchronous (replaces kernel.run(queue, (threads,), None, ...),
+ * argument value) on `stream` (a hipStream_t of the module's device; NULL =
+ * the default stream, and then the calling thread's current device must be
+ * the module's, else UFO_E_INVALID); asynchronous (replaces kernel.run(queue, (threads,), None, ...),
  * ufo/track.py:137). */
 int ufo_launch(ufo_module *mod, const char *kernel, const ufo_launch_cfg *cfg,
                void **args, size_t nargs, void *stream);

@@ -344,6 +344,13 @@ int ufo_launch(ufo_module *mod, const char *kernel, const ufo_launch_cfg *cfg, v
     if (nargs && !args) return fail(UFO_E_INVALID, "null argument list");
     for (int i = 0; i < 3; ++i)
         if (cfg->grid[i] == 0 || cfg->block[i] == 0) return fail(UFO_E_INVALID, "empty launch dimension");
+    if (!stream) {
+        int cur = -1;
+        HIP_TRY(hipGetDevice(&cur));
+        if (cur != mod->dev)
+            return fail(UFO_E_INVALID, "default-stream launch on device %d of a module loaded on device %d", cur,
+                        mod->dev);
+    }
     hipFunction_t f;
     int rc = lookup(mod, kernel, &f);
     if (rc) return rc;
