@@ -370,7 +370,7 @@ def adjoint_timing(cells=48, particles=65536, turns=512, every=32, warmup=2, rep
         alive = torch.isfinite(xf[:, 0])
         lam[alive] = 2.0 * xf[alive] * torch.tensor([1.0, 0.0, 1.0, 0.0], device=tr.device) / particles
         e[2].record(stream)
-        gx0, gth = tr.backward(lam)
+        tr.backward(lam)
         e[3].record(stream)
         sa.launch()
         e[4].record(stream)

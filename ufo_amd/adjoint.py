@@ -30,11 +30,10 @@ from __future__ import annotations
 import math
 from typing import Dict, List
 
-import numpy as np
 import torch
 
 from . import codegen, ir
-from .codegen import Env, _fma, _fma_neg, _negs, is_zero, emit_op, kick_coeffs
+from .codegen import Env, _fma, _fma_neg, is_zero, kick_coeffs
 from .flags import DOUBLE_PRECISION
 from .sim import Simulation
 

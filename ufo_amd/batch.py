@@ -36,8 +36,6 @@ import numpy as np
 import torch
 
 from . import codegen, ir, runtime
-from .codegen import Env
-from .flags import DOUBLE_PRECISION, FIVED
 from .sim import Simulation, host_buffer
 
 TILE = 64          # particles per tile and lane-particle (a tile is one wave's work)

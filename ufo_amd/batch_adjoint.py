@@ -174,7 +174,7 @@ def _reverse_turn(ops, env, diff_cols, period):
             adjoint.emit_reverse(op, env, body, g)
         decl = [f"ufloat {r} = 0.0;" for r in g.sums.values()]
         flush = []
-        for k, i in enumerate(g.params()):
+        for i in g.params():
             terms = [(r, dd) for (ii, dd), r in g.sums.items() if ii == i]
             acc_ = None
             for r, dd in terms:
@@ -253,7 +253,6 @@ def batched_adjoint_kernels(line, flags, turns, coords, lattice_parameters, diff
     fbody = [_fill(x, layout) for x in fbody]
     rbody = [_fill(x, layout) for x in rbody]
     inv_b = [_fill(x, layout) for x in envb.inv_code]
-    decl, flush = [], []
     gtab = ""
     if gmaps:
         rows = ", ".join("{" + ", ".join(str(v) for v in row) + "}" for row in gmaps)

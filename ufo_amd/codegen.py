@@ -1099,7 +1099,6 @@ def optics_kernel(line, flags, where, parameters, name="ufo_optics") -> Kernel:
     store = ("{ ufloat *o = tracks + ((size_t)ray * count + s) * 6; "
              "o[0] = x; o[1] = px; o[2] = y; o[3] = py; ++s; }")
     body, _ = _layout(ops, env, observe=store)
-    P = max(1, len(parameters))
     zero, one = env.lit(0.0), env.lit(1.0)
     lines = [f"x = y = z = px = py = dp = {zero};",
              f"if (ray == 0) {{ x = {one}; y = {one}; }}",

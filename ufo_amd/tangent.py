@@ -20,11 +20,10 @@ Kernels:
 """
 from __future__ import annotations
 
-import math
 from typing import List
 
 from . import codegen, ir
-from .codegen import Env, _fma, _fma_neg, _negs, is_zero, emit_op, kick_coeffs
+from .codegen import Env, _fma, _fma_neg, is_zero, emit_op, kick_coeffs
 
 def _loop(body: str) -> str:
     return f"_Pragma(\"unroll\") for (int k = 0; k < NT; ++k) {{ {body} }}"

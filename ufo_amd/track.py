@@ -2,7 +2,6 @@
 points every turn (ufo/track.py:19-139)."""
 from __future__ import annotations
 
-import numpy as np
 import torch
 
 from . import codegen
