@@ -78,3 +78,43 @@ def run_track(kernel, pool, samples, splits=()):
                      state.ctypes.data_as(ctypes.c_void_p), ctypes.c_int(n), ctypes.c_int(kernel.block),
                      ctypes.c_int(t0), ctypes.c_int(t1))
     return out.transpose(2, 0, 1)
+
+
+_SANITIZED_MAIN = r"""
+#include <cstdio>
+#include <cstdlib>
+int main() {
+    const int n = %(n)d, P = %(P)d, S = %(S)d;
+    // every buffer exactly as large as the launch needs, so that any access
+    // past an end is reported
+    ufloat *pool = (ufloat *)malloc(sizeof(ufloat) * n * P);
+    ufloat *tracks = (ufloat *)malloc(sizeof(ufloat) * S * 6 * n);
+    ufloat *state = (ufloat *)malloc(sizeof(ufloat) * 4 * n);
+    for (int i = 0; i < n * P; ++i) pool[i] = (ufloat)(1e-4 * ((i * 7919) %% 101 - 50));
+    for (int i = 0; i < S * 6 * n; ++i) tracks[i] = 0;
+    const int b[] = {%(bounds)s};
+    for (int k = 0; k + 1 < (int)(sizeof b / sizeof b[0]); ++k)
+        host_run(pool, tracks, state, n, %(block)d, b[k], b[k + 1]);
+    free(pool); free(tracks); free(state);
+    return 0;
+}
+"""
+
+
+def sanitized_run(kernel, particles, columns, splits=()):
+    """Build the emitted kernel, the launch loop and exactly sized buffers
+    as a host program under AddressSanitizer and UndefinedBehaviorSanitizer
+    and run it over the turns cut at ``splits``; returns (exit status,
+    sanitizer report)."""
+    samples = kernel.dumps_per_turn * kernel.turns
+    bounds = ", ".join(str(t) for t in [0] + sorted(splits) + [kernel.turns])
+    src = (_PRELUDE + kernel.src + (_DRIVER % kernel.name)
+           + _SANITIZED_MAIN % dict(n=particles, P=max(1, columns), S=samples, block=kernel.block, bounds=bounds))
+    d = tempfile.mkdtemp(prefix="ufo_sanitized_")
+    cpp, exe = os.path.join(d, "k.cpp"), os.path.join(d, "k")
+    with open(cpp, "w") as f:
+        f.write(src)
+    subprocess.run(["g++", "-std=c++17", "-O1", "-g", "-ffp-contract=off", "-fsanitize=address,undefined",
+                    "-fno-sanitize-recover=all", "-w", "-o", exe, cpp], check=True)
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    return r.returncode, r.stderr

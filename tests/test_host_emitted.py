@@ -92,3 +92,23 @@ def test_observation_point_in_every_cell_of_a_periodic_ring():
     r = compare_tracks(host, strict, fast, moved)
     print(summary(r))
     assert_parity(r)
+
+
+@pytest.mark.parametrize("flags,params,where,turns,splits,n", [
+    (ufo.KICK | ufo.DOUBLE_PRECISION, ["x", "px", "y", "py", "dp"], [-1], 20, [], 77),
+    (ufo.KICK | ufo.DOUBLE_PRECISION, ["x", "px", "y", "py", "dp"], [0, 3, 9, -1], 20, [7, 13], 300),
+    (ufo.KICK | ufo.DOUBLE_PRECISION | ufo.FIVED, ["x", "y"], [2, -1], 9, [1, 8], 65),
+    (ufo.KICK | ufo.EXACT | ufo.DOUBLE_PRECISION, ["x", "px", "y", "py", "dp"], [-1], 10, [5], 1),
+    (ufo.KICK, ["x", "px", "y", "py", "dp"], [1, -1], 10, [3], 129),
+    (ufo.DOUBLE_PRECISION, ["x", "px", "y", "py", "dp"], [-1, 4], 12, [6], 64),
+    (ufo.KICK | ufo.DOUBLE_PRECISION, [], [-1], 5, [], 4),
+])
+def test_emitted_track_kernel_stays_in_bounds(flags, params, where, turns, splits, n):
+    """The emitted Track kernel and its resumable launches, built for the
+    host under AddressSanitizer and UndefinedBehaviorSanitizer with every
+    buffer exactly as large as the launch needs: particle counts that do
+    not fill a block, several observation points, launches cut at arbitrary
+    turns, FIVED, EXACT, thick, FP32 and no parameter columns."""
+    k = codegen.track_kernel(lattices.fodo_ring(4), flags, turns, params, where, 0.0)
+    rc, report = host_emulation.sanitized_run(k, n, len(params), splits)
+    assert rc == 0, report[-2000:]
