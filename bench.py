@@ -50,6 +50,7 @@ def main():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-seed-adjoint", action="store_true")
     ap.add_argument("--no-fodo", action="store_true")
+    ap.add_argument("--no-co-optics", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="MiB copied by one device-to-device copy before the timed region "
                          "(reference kernel for HBM traffic counters)")
@@ -151,6 +152,9 @@ def main():
     if not args.no_fodo:
         out["fodo_track"] = fodo_track_timing(cpu=not args.no_cpu)
         _progress("FODO Track done")
+    if not args.no_co_optics:
+        out["co_optics"] = co_optics_timing(cpu=not args.no_cpu)
+        _progress("ClosedOrbit and Optics done")
     if not args.no_adjoint:
         out["adjoint"] = adjoint_timing()
         _progress("adjoint done")
@@ -226,6 +230,59 @@ def fodo_track_timing(particles=4096, turns=1000, reps=3, cpu=True):
         out["cpu_pet_per_s"] = pt * E / dt
         out["cpu_threads"] = threads
         out["cpu_kind"] = "port (oracle, whole workload)"
+    return out
+
+
+def co_optics_timing(reps=5, cpu=True):
+    """ClosedOrbit and Optics on the ALBA ring (768 elements, the lattice of
+    the reference's tests and HOWTO): 120 closed orbits, one per sextupole
+    displaced by dx = 1 mm (FIVED FP64; Newton on the tangent kernel here,
+    the reference's Nelder-Mead search, 200 iterations, in the oracle), and
+    the periodic Optics with an observation point at every element (LINEAR |
+    ACHROMATIC, FP64).  Each run() is one small launch plus copies and host
+    algebra, bound by launch latency rather than by the FP64 units; the GPU
+    figures are run() wall times, medians of ``reps`` after one warm-up."""
+    import numpy as np
+    import ufo_amd as ufo
+    lat = ufo.Lattice(path=os.path.join(ROOT, "tests", "golden", "lattices", "alba.mad"))
+    ring = lat.RING
+    flat = ring.flatten()
+    sext = ring.find(lambda e: isinstance(e, ufo.Sextupole))
+    prm = [(e, "dx") for e in sext]
+    co_flags = ufo.FIVED | ufo.DOUBLE_PRECISION
+    co = ufo.ClosedOrbit(ring, particles=len(sext), flags=co_flags, parameters=prm)
+    co.parameters[:] = np.identity(len(sext)) * 1e-3
+    where = list(range(len(flat)))
+    op_flags = ufo.LINEAR | ufo.ACHROMATIC | ufo.DOUBLE_PRECISION
+    op = ufo.Optics(ring, where=list(where), flags=op_flags)
+    co_ms, op_ms = [], []
+    for obj, times in ((co, co_ms), (op, op_ms)):
+        obj.run()
+        for _ in range(reps):
+            t = time.perf_counter()
+            obj.run()
+            times.append(1e3 * (time.perf_counter() - t))
+    out = {"workload": f"ALBA ring ({len(flat)} elements): {len(sext)} closed orbits (one sextupole displaced "
+                       f"by 1 mm each, FIVED FP64); periodic Optics at all {len(flat)} elements (FP64)",
+           "closed_orbit_ms": round(float(np.median(co_ms)), 3),
+           "newton_steps_max": int(np.max(co.steps)),
+           "optics_ms": round(float(np.median(op_ms)), 3),
+           "qx": float(op.qx), "qy": float(op.qy)}
+    if cpu:
+        from oracle import runner
+        pool = np.concatenate([np.zeros((len(sext), 0)), co.parameters], axis=1)
+        runner.closed_orbit(ring, pool[:1], flags=co_flags, parameters=prm)    # builds the oracle
+        t = time.perf_counter()
+        orbits = runner.closed_orbit(ring, pool, flags=co_flags, parameters=prm)
+        out["cpu_closed_orbit_ms"] = round(1e3 * (time.perf_counter() - t), 1)
+        out["closed_orbit_max_diff_vs_oracle"] = float(np.max(np.abs(orbits - co.orbits)))
+        ref = runner.Optics(ring, where=list(where), flags=op_flags)
+        t = time.perf_counter()
+        ref.run()
+        out["cpu_optics_ms"] = round(1e3 * (time.perf_counter() - t), 1)
+        out["optics_max_rel_diff_beta_vs_oracle"] = float(np.max(np.abs(np.asarray(op.bx) / np.asarray(ref.bx) - 1.0)))
+        out["cpu_threads"] = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
+        out["cpu_kind"] = "port (oracle, whole workload, build excluded)"
     return out
 
 

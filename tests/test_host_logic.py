@@ -78,3 +78,30 @@ def test_fp32_host_evaluation_rounds_every_step():
     expect = f32(f32(f32(p[0]) * f32(0.3)) / f32(7.0)) + f32(f32(p[1]) * f32(p[0]))
     assert ir.evaluate(e, p, None, True) == float(expect)
     assert ir.evaluate(e, p) == (p[0] * 0.3 / 7.0 + p[1] * p[0])
+
+
+@pytest.mark.parametrize("where", ["every", "quads"])
+def test_optics_host_algebra_matches_the_oracle(where):
+    """The host half of Optics.run() (periodic Twiss, dispersion and phase
+    advance at every observation point from the three rays' linear map),
+    fed the oracle's ALBA linear-map track, against the oracle's restatement
+    of ufo/optics.py:156-238 on the same track: tunes and Twiss functions
+    agree to rounding."""
+    import os
+    from oracle import runner
+    from conftest import OPTICS_DIR
+    ring = ufo.Lattice(path=os.path.join(OPTICS_DIR, "alba.mad")).RING
+    flat = ring.flatten()
+    idx = list(range(len(flat))) if where == "every" else ring.find(lambda e: isinstance(e, ufo.Quadrupole))
+    ref = runner.Optics(ring, where=list(idx), flags=ufo.LINEAR | ufo.ACHROMATIC | ufo.DOUBLE_PRECISION)
+    ref.run()
+    o = object.__new__(ufo.Optics)
+    o.periodic, o.count = True, ref.count
+    o.dy0 = o.dpy0 = 0.0    # as Optics.__init__ leaves them (no vertical dispersion source)
+    o.ax, o.ay, o.bx, o.by, o.dx, o.dy, o.dpx, o.dpy, o.mux, o.muy = (np.empty(ref.count) for _ in range(10))
+    o._twiss(np.asarray(ref.track, dtype=np.float64))
+    assert abs(o.qx - ref.qx) < 1e-14 and abs(o.qy - ref.qy) < 1e-14
+    for name in ("bx", "by", "ax", "ay", "dx", "dpx", "mux", "muy"):
+        a, b = getattr(o, name), np.asarray(getattr(ref, name))
+        scale = max(1.0, float(np.max(np.abs(b))))
+        assert np.max(np.abs(a - b)) < 1e-12 * scale, name
