@@ -29,14 +29,21 @@ STAT_FACTOR = 2.0
 STAT_FLOOR = 1e-14
 
 
+BOUND = 1.0    # metres / radians: beyond it a coordinate has left any aperture
+
+
 def rel_err(a, b, axis_rms=0):
     """Per-entry relative error with the rms of b along ``axis_rms`` as the
-    floor of the denominator (finite entries of b only)."""
+    floor of the denominator.  The rms is taken over the finite entries
+    within BOUND: a diverging particle (the Track loss test is NaN only, so
+    coordinates can grow to 1e300 first) would otherwise raise the floor,
+    or overflow it to inf, and hide every other particle's error."""
     a = np.asarray(a, dtype=np.float64)
     b = np.asarray(b, dtype=np.float64)
     fin = np.isfinite(b)
     bb = np.where(fin, b, 0.0)
-    rms = np.sqrt(np.mean(bb ** 2, axis=axis_rms, keepdims=True))
+    inside = np.where(fin & (np.abs(b) <= BOUND), b, 0.0)
+    rms = np.sqrt(np.mean(inside ** 2, axis=axis_rms, keepdims=True))
     rms = np.where(rms > 0, rms, 1.0)
     den = np.maximum(np.abs(bb), rms)
     err = np.abs(np.where(fin, a, 0.0) - bb) / den
