@@ -31,7 +31,9 @@ extern "C" {
 #define UFO_E_HIP (-2)       /* HIP runtime error */
 #define UFO_E_COMPILE (-3)   /* hipRTC compile error; the log holds the compiler output */
 #define UFO_E_NOTFOUND (-4)  /* no such device or kernel */
-#define UFO_E_IO (-5)        /* code-object cache I/O error */
+#define UFO_E_IO (-5)        /* reserved: the code-object cache is best effort (a failed
+                                 * write leaves the result valid; a cached object the
+                                 * runtime refuses is rebuilt) */
 
 typedef struct ufo_ctx ufo_ctx;
 typedef struct ufo_module ufo_module;

@@ -69,3 +69,19 @@ def test_run_without_a_device_raises():
                             parameters=["x"])
     with pytest.raises(runtime.UfoError, match="no HIP device"):
         sa.run()
+
+
+@pytest.mark.parametrize("damage", ["truncated", "garbage", "empty"])
+def test_damaged_cache_entry_is_rebuilt(monkeypatch, tmp_path, damage):
+    """A code-object cache entry that is cut short, overwritten or empty is
+    not handed to the loader: the source is compiled again and the entry
+    replaced (CPU: hipRTC needs no device)."""
+    from ufo_amd import runtime
+    monkeypatch.setenv("UFO_CACHE_DIR", str(tmp_path))
+    src = 'extern "C" __global__ void k(double *a) { a[threadIdx.x] *= 2.0; } // cache entry test'
+    good = runtime.compile_only(src)
+    (entry,) = tmp_path.glob("*.hsaco")
+    entry.write_bytes({"truncated": good[: len(good) // 2], "garbage": b"\x7fELF" + bytes(200),
+                       "empty": b""}[damage])
+    assert runtime.compile_only(src) == good
+    assert entry.read_bytes() == good

@@ -9,6 +9,7 @@
 #include <hip/hiprtc.h>
 
 #include <cerrno>
+#include <cstdint>
 #include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
@@ -85,6 +86,26 @@ bool read_file(const std::string &path, std::vector<char> &out) {
     return got == out.size();
 }
 
+// A cached code object is handed to the loader only if its headers fit in
+// the file: an ELF64 image whose program and section header tables end
+// within it (a truncated entry is rebuilt instead), or an offload bundle.
+bool plausible_code_object(const std::vector<char> &b) {
+    static const char bundle[] = "__CLANG_OFFLOAD_BUNDLE__";
+    if (b.size() >= sizeof bundle - 1 && memcmp(b.data(), bundle, sizeof bundle - 1) == 0) return true;
+    if (b.size() < 64 || memcmp(b.data(), "\x7f" "ELF", 4) != 0 || b[4] != 2) return false;
+    uint64_t phoff = 0, shoff = 0;
+    uint16_t phentsize = 0, phnum = 0, shentsize = 0, shnum = 0;
+    memcpy(&phoff, b.data() + 0x20, 8);
+    memcpy(&shoff, b.data() + 0x28, 8);
+    memcpy(&phentsize, b.data() + 0x36, 2);
+    memcpy(&phnum, b.data() + 0x38, 2);
+    memcpy(&shentsize, b.data() + 0x3A, 2);
+    memcpy(&shnum, b.data() + 0x3C, 2);
+    const uint64_t n = b.size();
+    return phoff <= n && shoff <= n && uint64_t(phnum) * phentsize <= n - phoff &&
+           uint64_t(shnum) * shentsize <= n - shoff;
+}
+
 void write_file_atomic(const std::string &path, const void *data, size_t n) {
     std::string tmp = path + ".tmp" + std::to_string(getpid());
     FILE *f = fopen(tmp.c_str(), "wb");
@@ -120,8 +141,10 @@ std::vector<std::string> split_options(const char *options) {
     return out;
 }
 
+// `cached` (optional) is set when the object came from the on-disk cache;
+// `refresh` skips the cache read and overwrites the entry.
 int compile_rtc(const char *src, const char *options, const std::string &arch,
-                std::vector<char> &code, std::string &log) {
+                std::vector<char> &code, std::string &log, bool *cached = nullptr, bool refresh = false) {
     int ver_major = 0, ver_minor = 0;
     hiprtcVersion(&ver_major, &ver_minor);
     std::string key_text = std::string(src) + '\0' + (options ? options : "") + '\0' + arch + '\0' +
@@ -132,7 +155,11 @@ int compile_rtc(const char *src, const char *options, const std::string &arch,
     std::string dir = cache_dir();
     mkdir(dir.c_str(), 0755);
     std::string path = dir + "/" + hex + ".hsaco";
-    if (read_file(path, code)) return UFO_OK;
+    if (cached) *cached = false;
+    if (!refresh && read_file(path, code) && plausible_code_object(code)) {
+        if (cached) *cached = true;
+        return UFO_OK;
+    }
 
     hiprtcProgram prog;
     hiprtcResult r = hiprtcCreateProgram(&prog, src, "ufo_kernel.hip", 0, nullptr, nullptr);
@@ -302,7 +329,16 @@ int ufo_module_compile(ufo_ctx *ctx, const char *src, const char *options, const
     if (!ctx || !src || !mod) return fail(UFO_E_INVALID, "null argument");
     std::vector<char> obj;
     std::string text;
-    int rc = compile_rtc(src, options, arch ? std::string(arch) : ctx->arch, obj, text);
+    std::string target = arch ? std::string(arch) : ctx->arch;
+    bool cached = false;
+    int rc = compile_rtc(src, options, target, obj, text, &cached);
+    copy_log(log, loglen, text);
+    if (rc) return rc;
+    rc = ufo_module_load(ctx, obj.data(), obj.size(), mod);
+    if (rc == UFO_OK || !cached) return rc;
+    // a cached object the runtime refuses (damaged, or written by another
+    // toolchain under the same key) is rebuilt once and replaced
+    rc = compile_rtc(src, options, target, obj, text, nullptr, true);
     copy_log(log, loglen, text);
     if (rc) return rc;
     return ufo_module_load(ctx, obj.data(), obj.size(), mod);
