@@ -120,3 +120,20 @@ def test_tangent_kernels_compile():
         runtime.compile_only(tangent.jvp_kernel(line, fl, 2, ["x", "px", "y", "py", "dp"], 1e-3).src,
                              codegen.HIPRTC_OPTIONS, "gfx950")
         runtime.compile_only(tangent.co_kernel(line, fl, ["x"], 20, 1e-15).src, codegen.HIPRTC_OPTIONS, "gfx950")
+
+
+def test_z_function_matches_brute_force():
+    """The linear-time longest-common-prefix table behind the period search
+    equals the direct definition on random sequences over small alphabets
+    (long repeats) and on periodic ones."""
+    from ufo_amd.codegen import _z_function
+    rng = np.random.default_rng(5)
+    seqs = [list(rng.integers(0, k, n)) for k in (1, 2, 3, 7) for n in (1, 2, 5, 40, 257)]
+    seqs += [[0, 1, 2] * 30 + [0, 1], [4, 4, 5] * 17 + [9]]
+    for s in seqs:
+        z = _z_function(s)
+        for i in range(1, len(s)):
+            k = 0
+            while i + k < len(s) and s[k] == s[i + k]:
+                k += 1
+            assert z[i] == k, (s, i)

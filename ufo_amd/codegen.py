@@ -707,6 +707,22 @@ def _param_maps(par, head, p, reps, fixed_below):
     return maps
 
 
+def _z_function(s):
+    """z[i] = length of the longest common prefix of s and s[i:] (z[0] = 0),
+    in linear time (Gusfield's Z algorithm)."""
+    n = len(s)
+    z = [0] * n
+    lo = hi = 0
+    for i in range(1, n):
+        k = min(hi - i, z[i - lo]) if i < hi else 0
+        while i + k < n and s[k] == s[i + k]:
+            k += 1
+        z[i] = k
+        if i + k > hi:
+            lo, hi = i, i + k
+    return z
+
+
 def find_period(ops, with_stride=False, remap_from=None):
     """Largest coverage (head, period, repeats[, stride]) of the op list by a
     repeated block starting within the first few ops.  Repetitions may
@@ -717,22 +733,23 @@ def find_period(ops, with_stride=False, remap_from=None):
     none = (0, 0, 0, 0) if with_stride else (0, 0, 0)
     if n < 2:
         return none
-    ids = {}
+    ids, seen = {}, {}
     sigs, par = [], []
     for op in ops:
-        sg, pr = _signature(op)
-        sigs.append(ids.setdefault(sg, len(ids)))
-        par.append(pr)
-    seq = np.array(sigs, dtype=np.int64)
+        r = seen.get(op)
+        if r is None:    # identical ops (the cells of a ring) are abstracted once
+            sg, pr = _signature(op)
+            r = seen[op] = (ids.setdefault(sg, len(ids)), pr)
+        sigs.append(r[0])
+        par.append(r[1])
     cands = []
     for head in range(0, min(9, n - 1)):
-        s = seq[head:]
-        m = len(s)
-        for p in range(1, m // 2 + 1):
-            eq = s[p:] == s[:-p]
-            bad = np.flatnonzero(~eq)
-            run = (len(eq) if len(bad) == 0 else bad[0]) + p
-            reps = run // p
+        s = sigs[head:]
+        z = _z_function(s)
+        for p in range(1, len(s) // 2 + 1):
+            # s[head + p:] repeats s[head:] for z[p] ops: the block of p ops
+            # runs (z[p] + p) // p times
+            reps = (z[p] + p) // p
             if reps >= 2:
                 cands.append((reps * p, -head, -p, head, p, reps))
     cands.sort(reverse=True)
@@ -951,7 +968,13 @@ def _decls(env: Env):
 
 
 def _count(ops, env):
-    flops = sum(ir.op_flops(op.map_exprs(env.f)) for op in ops)
+    per_op = {}    # identical ops (the cells of a ring) are folded once
+    flops = 0
+    for op in ops:
+        f = per_op.get(op)
+        if f is None:
+            f = per_op[op] = ir.op_flops(op.map_exprs(env.f))
+        flops += f
     return flops
 
 
