@@ -266,6 +266,34 @@ def test_optics_alba_tunes():
         np.testing.assert_allclose(getattr(o, a), getattr(r, a), rtol=1e-9, atol=1e-12)
 
 
+
+def test_optics_alba_default_fp32():
+    """tests/optics.py, chroma.py and radiation.py run Optics with its
+    default flags (LINEAR | ACHROMATIC, FP32).  HIP in FP32: tunes within
+    the reference test's 1e-5 of MAD-X; Twiss functions and chromaticity
+    within twice the reference's own FP32 spread (its strict against its
+    fast-math build) of the strict oracle, plus 1e-6."""
+    alba = ufo.Lattice(os.path.join(OPTICS_DIR, "alba.mad"))
+    where = alba.RING.find(lambda e: hasattr(e, "k1") or hasattr(e, "k2"))
+    o = ufo.Optics(alba.RING, where=list(where))
+    o.run(threads=1)
+    assert o.fp32
+    assert abs(o.qx - 0.15543311) < 1e-5 and abs(o.qy - 0.362008337) < 1e-5
+    strict = runner.Optics(alba.RING, where=list(where), flags=ufo.LINEAR | ufo.ACHROMATIC)
+    fast = runner.Optics(alba.RING, where=list(where), flags=ufo.LINEAR | ufo.ACHROMATIC, mode="fast")
+    strict.run()
+    fast.run()
+    for a in ("bx", "by", "ax", "ay", "dx", "dpx"):
+        hip, ref, alt = (np.asarray(getattr(x, a), dtype=np.float64) for x in (o, strict, fast))
+        scale = max(1.0, float(np.max(np.abs(ref))))
+        spread = float(np.max(np.abs(alt - ref))) / scale
+        err = float(np.max(np.abs(hip - ref))) / scale
+        print(f"{a}: hip {err:.2e}, reference spread {spread:.2e}")
+        assert err <= 2.0 * spread + 1e-6, a
+    (n_hip, t_hip), (n_ref, t_ref), (n_alt, t_alt) = (ufo.chromaticity(x) for x in (o, strict, fast))
+    for h, r, a in zip(n_hip + t_hip, n_ref + t_ref, n_alt + t_alt):
+        assert abs(h - r) <= 2.0 * abs(a - r) + 1e-6 * max(1.0, abs(r))
+
 def test_optics_consumers_alba():
     """chromaticity, emittance and rdt (ufo/chroma.py, ufo/radiation.py,
     ufo/rdt.py) on the HIP Optics of ALBA agree with the same functions on
