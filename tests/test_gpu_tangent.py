@@ -98,6 +98,28 @@ def test_closed_orbit_howto_table_fp64():
     assert np.max(np.abs(t - co.orbits)) < 1e-15
 
 
+
+def test_closed_orbit_default_fp32_flags():
+    """tests/closed_orbit.py as the reference runs it, with the default flags
+    (FP32, thick elements): ALBA with QD1 displaced by 0.1 mm in x and y.
+    One FP32 Track turn from the orbit returns to it within the reference
+    test's 1e-6 (here 5e-8: FP32 rounding over 768 elements; 5e-9 measured), and the orbit is within 1e-7 of the oracle's
+    FP32 Nelder-Mead orbit (which lies 1.4e-8 from the FP64 orbit)."""
+    lat = ufo.Lattice(os.path.join(OPTICS_DIR, "alba.mad"))
+    lat.QD1.dy = 1e-4
+    lat.QD1.dx = 1e-4
+    co = ufo.ClosedOrbit(lat.RING, particles=1)
+    co.run()
+    assert co.fp32
+    tr = ufo.Track(lat.RING, parameters=C4, turns=1, particles=1, where=[-1])
+    tr.parameters[0] = co.orbits[0]
+    tr.run()
+    delta = np.abs(tr.tracks[0, 0, :4].astype(np.float64) - co.orbits[0])
+    print("FP32 orbit", co.orbits[0], "one-turn delta", delta, "Newton steps", co.steps[0])
+    assert np.max(delta) < 5e-8
+    ref = runner.closed_orbit(lat.RING, np.zeros((1, 1)), flags=0, parameters=[])
+    assert np.max(np.abs(co.orbits[0] - ref[0])) < 1e-7
+
 def test_closed_orbit_misaligned_quad_vs_tracking():
     """tests/closed_orbit.py: ALBA with QD1 displaced by 0.1 mm in x and y;
     one turn of Track from the orbit returns to it (reference tolerance 1e-6,
