@@ -22,7 +22,8 @@ class ClosedOrbit(Simulation):
     particle's parameter set.  Same arguments and attributes as the reference
     (ufo/co.py:146-250): ``parameters`` [particles, max(1, P)], ``orbits``
     [particles, 4] after ``run()``.  ``iterations`` bounds the Newton steps;
-    ``tol`` is the stopping residual max |F(X) - X| in metres/radians.
+    ``tol`` is the stopping residual max |F(X) - X| in metres/radians (in
+    FP32, Newton also stops once a step no longer lowers the residual).
     ``steps`` holds the Newton steps each particle took."""
 
     def __init__(self, line, flags=0x00, particles=1000, parameters=(), dp=0., iterations=200,

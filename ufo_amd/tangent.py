@@ -232,7 +232,8 @@ def co_kernel(line, flags, parameters, iterations, tol, block=64, name="ufo_co")
     the particle's x, px, y, py parameters (0 when not named), as the
     reference seeds its simplex (ufo/co.py:208-213).  Output [particles][4];
     an orbit that is lost or does not converge is returned as it stands
-    after ``iterations`` steps."""
+    after ``iterations`` steps.  In FP32 the iteration also ends at the first
+    step that does not lower the residual, returning the best iterate."""
     pieces = line.method(flags=flags, parameters=parameters)
     ops = [op for piece in pieces for op in piece]
     src = codegen._coord_sources(parameters, 0, 0.0)
@@ -245,6 +246,12 @@ def co_kernel(line, flags, parameters, iterations, tol, block=64, name="ufo_co")
     one, zero = env.lit(1.0), env.lit(0.0)
     code = codegen._PRELUDE.format(ufloat="float" if env.fp32 else "double",
                                    nan='__builtin_nanf("")' if env.fp32 else '__builtin_nan("")')
+    # FP32 cannot reach a 1e-15 residual: stop at the first step that does
+    # not lower it and keep the best iterate (FP64 runs to tol or the cap)
+    stall = ("""
+        if (!(rmax < best)) { X[0] = XB[0]; X[1] = XB[1]; X[2] = XB[2]; X[3] = XB[3]; break; }
+        best = rmax; XB[0] = X[0]; XB[1] = X[1]; XB[2] = X[2]; XB[3] = X[3];""" if env.fp32 else "")
+    best_decl = "\n    ufloat best = __builtin_huge_valf(), XB[4] = {x, px, y, py};" if env.fp32 else ""
     code += "#define NT 4\n" + _SOLVE4
     code += f"""
 extern "C" __global__ void __launch_bounds__({block})
@@ -256,7 +263,7 @@ extern "C" __global__ void __launch_bounds__({block})
     if (idx >= particles) return;
     const ufloat *__restrict__ row = pool + (size_t)idx * {P};
 {chr(10).join(load)}
-    ufloat X[4] = {{x, px, y, py}};
+    ufloat X[4] = {{x, px, y, py}};{best_decl}
     int it = 0;
     for (; it < {int(iterations)}; ++it) {{
         x = X[0]; px = X[1]; y = X[2]; py = X[3];
@@ -272,7 +279,7 @@ extern "C" __global__ void __launch_bounds__({block})
         ufloat r[4] = {{x - X[0], px - X[1], y - X[2], py - X[3]}};
         if (isnan(x) || isnan(px) || isnan(y) || isnan(py)) break;
         ufloat rmax = fmax(fmax(fabs(r[0]), fabs(r[1])), fmax(fabs(r[2]), fabs(r[3])));
-        if (rmax <= {env.lit(tol)}) break;
+        if (rmax <= {env.lit(tol)}) break;{stall}
         ufloat A[4][4];
 #pragma unroll
         for (int k = 0; k < NT; ++k) {{
