@@ -103,7 +103,7 @@ def test_track_run_split_over_launches(double):
         np.testing.assert_array_equal(r.view(u), runs[0].view(u))
 
 
-def test_fodo_config0_full_size():
+def test_fodo16_4096_particles_1000_turns_full_size():
     """The reference's own CPU workload at full size: FODO-16, 4096
     particles (seed 0), 6D, 1000 turns, where=[-1].  After 1000 turns the
     reference's arithmetic disagrees with itself at the 4e-12 level (median
