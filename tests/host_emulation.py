@@ -118,3 +118,53 @@ def sanitized_run(kernel, particles, columns, splits=()):
                     "-fno-sanitize-recover=all", "-w", "-o", exe, cpp], check=True)
     r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
     return r.returncode, r.stderr
+
+
+# buffers and call of the other kernel kinds: (allocations, call, frees);
+# n = particles (rows of pool), S = samples per ray (Optics)
+_KINDS = {
+    "optics": ("ufloat *tracks = (ufloat *)malloc(sizeof(ufloat) * 3 * S * 6);", "%s(pool, tracks, S);",
+               "free(tracks);"),
+    "jvp": ("ufloat *out = (ufloat *)malloc(sizeof(ufloat) * n * 20);", "%s(pool, out, n);", "free(out);"),
+    "co": ("ufloat *orbits = (ufloat *)malloc(sizeof(ufloat) * n * 4); int *steps = (int *)malloc(sizeof(int) * n);",
+           "%s(pool, orbits, steps, n);", "free(orbits); free(steps);"),
+}
+
+_KIND_MAIN = r"""
+#include <cstdio>
+#include <cstdlib>
+int main() {
+    const int n = %(n)d, P = %(P)d, S = %(S)d, blocks = %(blocks)d, block = %(block)d;
+    ufloat *pool = (ufloat *)malloc(sizeof(ufloat) * n * P);
+    for (int i = 0; i < n * P; ++i) pool[i] = (ufloat)(1e-5 * ((i * 7919) %% 101 - 50));
+    %(alloc)s
+    for (int b = 0; b < blocks; ++b) {
+        blockIdx.x = b;
+        for (int t = 0; t < block; ++t) { threadIdx.x = t; %(call)s }
+    }
+    %(free)s
+    free(pool);
+    (void)S;
+    return 0;
+}
+"""
+
+
+def sanitized_run_kind(kernel, particles, columns, samples=0):
+    """``sanitized_run`` for the Optics (one 64-lane block, three rays,
+    ``samples`` per ray), JVP and ClosedOrbit kernels: every buffer exactly
+    as large as the launch needs; returns (exit status, sanitizer report)."""
+    alloc, call, free = _KINDS[kernel.kind]
+    n = 1 if kernel.kind == "optics" else particles
+    blocks = 1 if kernel.kind == "optics" else (particles + kernel.block - 1) // kernel.block
+    src = (_PRELUDE + kernel.src
+           + _KIND_MAIN % dict(n=n, P=max(1, columns), S=samples, blocks=blocks, block=kernel.block, alloc=alloc,
+                               call=call % kernel.name, free=free))
+    d = tempfile.mkdtemp(prefix="ufo_sanitized_")
+    cpp, exe = os.path.join(d, "k.cpp"), os.path.join(d, "k")
+    with open(cpp, "w") as f:
+        f.write(src)
+    subprocess.run(["g++", "-std=c++17", "-O1", "-g", "-ffp-contract=off", "-fsanitize=address,undefined",
+                    "-fno-sanitize-recover=all", "-w", "-o", exe, cpp], check=True)
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    return r.returncode, r.stderr

@@ -112,3 +112,29 @@ def test_emitted_track_kernel_stays_in_bounds(flags, params, where, turns, split
     k = codegen.track_kernel(lattices.fodo_ring(4), flags, turns, params, where, 0.0)
     rc, report = host_emulation.sanitized_run(k, n, len(params), splits)
     assert rc == 0, report[-2000:]
+
+
+
+@pytest.mark.parametrize("kind,double", [("optics", True), ("optics", False), ("jvp", True), ("co", True),
+                                         ("co", False)])
+def test_emitted_optics_tangent_and_orbit_kernels_stay_in_bounds(kind, double):
+    """The Optics (fractional and end-of-line observation points), JVP and
+    ClosedOrbit kernels, built for the host under AddressSanitizer and
+    UndefinedBehaviorSanitizer with exactly sized buffers and particle
+    counts that do not fill a block."""
+    from ufo_amd import tangent
+    ring = lattices.fodo_ring(3)
+    fl = ufo.DOUBLE_PRECISION if double else 0
+    samples = 0
+    if kind == "optics":
+        k = codegen.optics_kernel(ring, ufo.LINEAR | ufo.ACHROMATIC | fl, [0.5, 2, 4.25, -1], [])
+        samples = max(4, k.dumps_per_turn)
+        cols, n = 1, 1
+    elif kind == "jvp":
+        cols, n = 5, 77
+        k = tangent.jvp_kernel(ring, ufo.KICK | fl, 3, ["x", "px", "y", "py", "dp"], 0.0)
+    else:
+        cols, n = 2, 70
+        k = tangent.co_kernel(ring, fl, [(1, "dx"), (3, "dy")], 8, 1e-15)
+    rc, report = host_emulation.sanitized_run_kind(k, n, cols, samples)
+    assert rc == 0, report[-2000:]
