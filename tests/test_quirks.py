@@ -5,7 +5,7 @@ import numpy as np
 import pytest
 
 import ufo_amd as ufo
-from ufo_amd import codegen, ir
+from ufo_amd import codegen, ir, lattices
 from oracle import runner
 
 import host_emulation
@@ -91,3 +91,25 @@ def test_nan_start_is_lost_at_turn_zero():
     ref = runner.track(line, pool, flags=FL, turns=6, parameters=COORD6, where=[-1])
     np.testing.assert_array_equal(np.isnan(host[:, :, 0]), np.isnan(ref[:, :, 0]))
     assert np.all(np.isnan(host[1, :, 0])) and not np.any(np.isnan(host[0, :, 0]))
+
+
+def test_duplicate_observation_point_loses_the_particle_as_in_the_reference():
+    """where = [3, 3, -1] on a KICK ring whose element 3 is a quadrupole: the
+    reference fractures the element at 0 a second time (ufo/lattice.py:101-104
+    removes one 0 and passes the other to the element), and the zero-length
+    teapot piece evaluates its weak-focusing term as 0/0 (ufo/methods.py), so
+    px turns NaN at the duplicate point and the particle is lost in turn 0.
+    The emitted kernel does the same: px is NaN in the second sample, every
+    later sample is NaN and the loss flags agree with the oracle.  (The
+    oracle's strict build also carries the NaN into x through the
+    zero-length drift, px * 0; the kernels drop zero terms, as a compiler
+    under -cl-fast-relaxed-math may, so there x turns NaN at the next drift.)"""
+    line = lattices.fodo_ring(4)
+    assert isinstance(line.flatten()[3], ufo.Quadrupole)
+    pool = _bunch(8)
+    k = codegen.track_kernel(line, FL, 2, COORD6, [3, 3, -1], 0.0)
+    host = host_emulation.run_track(k, pool, 6)
+    ref = runner.track(line, pool, flags=FL, turns=2, parameters=COORD6, where=[3, 3, -1])
+    assert np.all(np.isnan(host[:, 1, 1])) and np.all(np.isnan(ref[:, 1, 1]))
+    assert np.all(np.isnan(host[:, 2:, :4])) and np.all(np.isnan(ref[:, 2:, :4]))
+    np.testing.assert_allclose(host[:, 0], ref[:, 0], rtol=1e-12, atol=1e-18)
