@@ -105,3 +105,18 @@ def test_optics_host_algebra_matches_the_oracle(where):
         a, b = getattr(o, name), np.asarray(getattr(ref, name))
         scale = max(1.0, float(np.max(np.abs(b))))
         assert np.max(np.abs(a - b)) < 1e-12 * scale, name
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU: --gpus above 1 exits with status 2 and a
+    JSON line saying so, before touching a device."""
+    import json
+    import os
+    import subprocess
+    import sys
+    from conftest import ROOT
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2
+    line = json.loads(r.stdout.strip().splitlines()[-1])
+    assert line["n_gpus"] == 2 and "one GPU" in line["error"]
