@@ -231,9 +231,9 @@ def co_kernel(line, flags, parameters, iterations, tol, block=64, name="ufo_co")
     dp = 0 (ufo/co.py:175-193 fixes dp = 0, oodppo = 1).  The initial guess is
     the particle's x, px, y, py parameters (0 when not named), as the
     reference seeds its simplex (ufo/co.py:208-213).  Output [particles][4];
-    an orbit that is lost or does not converge is returned as it stands
-    after ``iterations`` steps.  In FP32 the iteration also ends at the first
-    step that does not lower the residual, returning the best iterate."""
+    an orbit that is lost or does not reach ``tol`` within ``iterations``
+    steps is returned as the iterate with the smallest residual.  In FP32 the
+    iteration also ends at the first step that does not lower the residual."""
     pieces = line.method(flags=flags, parameters=parameters)
     ops = [op for piece in pieces for op in piece]
     src = codegen._coord_sources(parameters, 0, 0.0)
@@ -246,12 +246,14 @@ def co_kernel(line, flags, parameters, iterations, tol, block=64, name="ufo_co")
     one, zero = env.lit(1.0), env.lit(0.0)
     code = codegen._PRELUDE.format(ufloat="float" if env.fp32 else "double",
                                    nan='__builtin_nanf("")' if env.fp32 else '__builtin_nan("")')
-    # FP32 cannot reach a 1e-15 residual: stop at the first step that does
-    # not lower it and keep the best iterate (FP64 runs to tol or the cap)
-    stall = ("""
-        if (!(rmax < best)) { X[0] = XB[0]; X[1] = XB[1]; X[2] = XB[2]; X[3] = XB[3]; break; }
-        best = rmax; XB[0] = X[0]; XB[1] = X[1]; XB[2] = X[2]; XB[3] = X[3];""" if env.fp32 else "")
-    best_decl = "\n    ufloat best = __builtin_huge_valf(), XB[4] = {x, px, y, py};" if env.fp32 else ""
+    # Every precision keeps the best evaluated iterate and returns it when
+    # the residual never reaches tol (cap or loss); FP32, which cannot reach
+    # a 1e-15 residual, also stops at the first step that does not lower it.
+    huge = "__builtin_huge_valf()" if env.fp32 else "__builtin_huge_val()"
+    keep = ("""
+        if (!(rmax < best)) break;
+        best = rmax; XB[0] = X[0]; XB[1] = X[1]; XB[2] = X[2]; XB[3] = X[3];""" if env.fp32 else """
+        if (rmax < best) { best = rmax; XB[0] = X[0]; XB[1] = X[1]; XB[2] = X[2]; XB[3] = X[3]; }""")
     code += "#define NT 4\n" + _SOLVE4
     code += f"""
 extern "C" __global__ void __launch_bounds__({block})
@@ -263,7 +265,9 @@ extern "C" __global__ void __launch_bounds__({block})
     if (idx >= particles) return;
     const ufloat *__restrict__ row = pool + (size_t)idx * {P};
 {chr(10).join(load)}
-    ufloat X[4] = {{x, px, y, py}};{best_decl}
+    ufloat X[4] = {{x, px, y, py}};
+    ufloat best = {huge}, XB[4] = {{x, px, y, py}};
+    bool conv = false;
     int it = 0;
     for (; it < {int(iterations)}; ++it) {{
         x = X[0]; px = X[1]; y = X[2]; py = X[3];
@@ -279,7 +283,7 @@ extern "C" __global__ void __launch_bounds__({block})
         ufloat r[4] = {{x - X[0], px - X[1], y - X[2], py - X[3]}};
         if (isnan(x) || isnan(px) || isnan(y) || isnan(py)) break;
         ufloat rmax = fmax(fmax(fabs(r[0]), fabs(r[1])), fmax(fabs(r[2]), fabs(r[3])));
-        if (rmax <= {env.lit(tol)}) break;{stall}
+        if (rmax <= {env.lit(tol)}) {{ conv = true; break; }}{keep}
         ufloat A[4][4];
 #pragma unroll
         for (int k = 0; k < NT; ++k) {{
@@ -291,6 +295,7 @@ extern "C" __global__ void __launch_bounds__({block})
 #pragma unroll
         for (int i = 0; i < 4; ++i) X[i] -= r[i];
     }}
+    if (!conv) {{ X[0] = XB[0]; X[1] = XB[1]; X[2] = XB[2]; X[3] = XB[3]; }}
     ufloat *o = orbits + (size_t)idx * 4;
     o[0] = X[0]; o[1] = X[1]; o[2] = X[2]; o[3] = X[3];
     steps[idx] = it;
