@@ -23,7 +23,9 @@ class ClosedOrbit(Simulation):
     (ufo/co.py:146-250): ``parameters`` [particles, max(1, P)], ``orbits``
     [particles, 4] after ``run()``.  ``iterations`` bounds the Newton steps;
     ``tol`` is the stopping residual max |F(X) - X| in metres/radians (in
-    FP32, Newton also stops once a step no longer lowers the residual).
+    FP32, Newton also stops once a step no longer lowers the residual); an
+    orbit that does not reach it is returned as the iterate with the
+    smallest residual.
     ``steps`` holds the Newton steps each particle took."""
 
     def __init__(self, line, flags=0x00, particles=1000, parameters=(), dp=0., iterations=200,
