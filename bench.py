@@ -270,7 +270,7 @@ def co_optics_timing(reps=5, cpu=True):
            "qx": float(op.qx), "qy": float(op.qy)}
     if cpu:
         from oracle import runner
-        pool = np.concatenate([np.zeros((len(sext), 0)), co.parameters], axis=1)
+        pool = np.array(co.parameters, dtype=np.float64)
         runner.closed_orbit(ring, pool[:1], flags=co_flags, parameters=prm)    # builds the oracle
         t = time.perf_counter()
         orbits = runner.closed_orbit(ring, pool, flags=co_flags, parameters=prm)
