@@ -110,6 +110,29 @@ int ufo_kernel_info(ufo_module *mod, const char *kernel, int *vgprs, int *sgprs,
 int ufo_launch(ufo_module *mod, const char *kernel, const ufo_launch_cfg *cfg,
                void **args, size_t nargs, void *stream);
 
+/* Largest scratch area ufo_launch_noalloc supplies, in bytes (1 MiB per
+ * device, held from the library's load on that device until the process
+ * ends). */
+#define UFO_SCRATCH_MAX_BYTES (1u << 20)
+
+/* ufo_launch for callers that cannot allocate device memory (code under
+ * graph capture, serving loops, harnesses that forbid hipMalloc).  No
+ * counterpart in the reference.  The kernel's scratch buffer (the work-pool
+ * counter of the aperture kernel, ufo/sa.py:122, or any other zero-initialised
+ * counters or staging of at most UFO_SCRATCH_MAX_BYTES) comes from a
+ * device-resident array of the library instead of the caller: args[scratch_arg]
+ * is ignored (may be NULL) and the kernel receives that array's device
+ * address there; its first scratch_bytes (rounded up to 4) are zeroed by a
+ * kernel on `stream` before the launch.  Inputs and outputs are the caller's,
+ * as for ufo_launch.  Makes no hipMalloc/hipFree and no host allocation;
+ * UFO_E_INVALID if scratch_bytes is 0 or above UFO_SCRATCH_MAX_BYTES, if
+ * scratch_arg >= nargs or nargs > 64.  Every call on one device shares that
+ * device's array, which keeps the last call's contents: calls that overlap on
+ * one device (other streams, other threads) race; serialise them. */
+int ufo_launch_noalloc(ufo_module *mod, const char *kernel, const ufo_launch_cfg *cfg,
+                       void **args, size_t nargs, size_t scratch_arg, size_t scratch_bytes,
+                       void *stream);
+
 /* Wait for all work on `stream` (NULL = whole device). */
 int ufo_sync(void *stream);
 

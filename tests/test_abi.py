@@ -85,3 +85,18 @@ def test_damaged_cache_entry_is_rebuilt(monkeypatch, tmp_path, damage):
                        "empty": b""}[damage])
     assert runtime.compile_only(src) == good
     assert entry.read_bytes() == good
+
+
+def test_noalloc_launch_argument_errors():
+    """ufo_launch_noalloc refuses a scratch size outside 1..UFO_SCRATCH_MAX_BYTES
+    and a scratch slot outside the argument list before it touches a device."""
+    from ufo_amd import runtime
+    lib = runtime.lib()
+    assert lib.ufo_launch_noalloc(None, b"k", None, None, 0, 0, (1 << 20) + 1, None) == -1
+    assert b"scratch of 1048577 bytes" in lib.ufo_last_error()
+    assert lib.ufo_launch_noalloc(None, b"k", None, None, 0, 0, 0, None) == -1
+    assert b"scratch of 0 bytes" in lib.ufo_last_error()
+    assert lib.ufo_launch_noalloc(None, b"k", None, None, 0, 0, 4, None) == -1
+    assert b"null argument" in lib.ufo_last_error()
+    header = open(os.path.join(ROOT, "include", "ufo_hip.h")).read()
+    assert "#define UFO_SCRATCH_MAX_BYTES (1u << 20)" in header

@@ -202,6 +202,48 @@ std::string base_arch(const char *gcn) {
 
 }  // namespace
 
+// Scratch memory of ufo_launch_noalloc: one array per device, allocated by
+// the HIP runtime when the library's code object is loaded on that device
+// and held until the process ends.  In host code `scratch` names a host-side
+// stand-in, never the device memory: the device address is looked up once per
+// device with hipGetSymbolAddress (scratch_address) and only that address is
+// handed to kernels and fills.
+static __device__ __align__(256) unsigned char scratch[UFO_SCRATCH_MAX_BYTES];
+
+// Zero the first n 32-bit words of p (a kernel, not hipMemsetAsync, so that a
+// caller may capture the call into a graph).
+__global__ void ufo_scratch_clear(uint32_t *p, unsigned int n) {
+    for (unsigned int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0u;
+}
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_lock;
+void *g_scratch_ptr[kMaxDevices];   // device address of `scratch` per device, guarded by g_scratch_lock
+
+// Device address of `scratch` on device `dev`, looked up on first use with
+// `dev` current (the caller's current device is restored).
+int scratch_address(int dev, void **out) {
+    if (dev < 0 || dev >= kMaxDevices) return fail(UFO_E_INVALID, "device %d beyond the scratch table", dev);
+    std::lock_guard<std::mutex> guard(g_scratch_lock);
+    if (!g_scratch_ptr[dev]) {
+        int cur = -1;
+        HIP_TRY(hipGetDevice(&cur));
+        if (cur != dev) HIP_TRY(hipSetDevice(dev));
+        void *ptr = nullptr;
+        hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (cur != dev) (void)hipSetDevice(cur);
+        if (e != hipSuccess || !ptr)
+            return fail(UFO_E_HIP, "hipGetSymbolAddress(scratch) on device %d: %s", dev, hipGetErrorString(e));
+        g_scratch_ptr[dev] = ptr;
+    }
+    *out = g_scratch_ptr[dev];
+    return UFO_OK;
+}
+
+}  // namespace
+
 struct ufo_ctx {
     int dev;
     std::string arch;
@@ -231,6 +273,24 @@ static int lookup(ufo_module *m, const char *kernel, hipFunction_t *f) {
     m->funcs[kernel] = *f;
     return UFO_OK;
 }
+
+// As lookup, without adding to the module's table (ufo_launch_noalloc makes
+// no host allocation either).
+static int find_function(ufo_module *m, const char *kernel, hipFunction_t *f) {
+    {
+        std::lock_guard<std::mutex> guard(m->lock);
+        auto it = m->funcs.find(kernel);
+        if (it != m->funcs.end()) {
+            *f = it->second;
+            return UFO_OK;
+        }
+    }
+    hipError_t e = hipModuleGetFunction(f, m->mod, kernel);
+    if (e != hipSuccess) return fail(UFO_E_NOTFOUND, "kernel '%s' not found: %s", kernel, hipGetErrorString(e));
+    return UFO_OK;
+}
+
+constexpr int kMaxNoallocArgs = 64;
 
 extern "C" {
 
@@ -284,6 +344,10 @@ int ufo_context_create(int dev, ufo_ctx **ctx) {
     if (!ctx) return fail(UFO_E_INVALID, "null argument");
     ufo_device_props p;
     int rc = ufo_device_props_get(dev, &p);
+    if (rc) return rc;
+    // set-up of the device's scratch array for ufo_launch_noalloc
+    void *ptr = nullptr;
+    rc = scratch_address(dev, &ptr);
     if (rc) return rc;
     *ctx = new ufo_ctx{dev, base_arch(p.arch)};
     return UFO_OK;
@@ -392,6 +456,46 @@ int ufo_launch(ufo_module *mod, const char *kernel, const ufo_launch_cfg *cfg, v
     if (rc) return rc;
     HIP_TRY(hipModuleLaunchKernel(f, cfg->grid[0], cfg->grid[1], cfg->grid[2], cfg->block[0], cfg->block[1],
                                   cfg->block[2], cfg->shared_bytes, static_cast<hipStream_t>(stream), args,
+                                  nullptr));
+    return UFO_OK;
+}
+
+int ufo_launch_noalloc(ufo_module *mod, const char *kernel, const ufo_launch_cfg *cfg, void **args, size_t nargs,
+                       size_t scratch_arg, size_t scratch_bytes, void *stream) {
+    if (scratch_bytes == 0 || scratch_bytes > UFO_SCRATCH_MAX_BYTES)
+        return fail(UFO_E_INVALID, "scratch of %zu bytes requested; the library holds 1 to %u bytes per device",
+                    scratch_bytes, (unsigned)UFO_SCRATCH_MAX_BYTES);
+    if (!mod || !kernel || !cfg) return fail(UFO_E_INVALID, "null argument");
+    if (!args || scratch_arg >= nargs)
+        return fail(UFO_E_INVALID, "scratch argument %zu outside the %zu kernel arguments", scratch_arg, nargs);
+    if (nargs > kMaxNoallocArgs) return fail(UFO_E_INVALID, "%zu kernel arguments (at most %d)", nargs, kMaxNoallocArgs);
+    for (int i = 0; i < 3; ++i)
+        if (cfg->grid[i] == 0 || cfg->block[i] == 0) return fail(UFO_E_INVALID, "empty launch dimension");
+    void *ptr = nullptr;
+    int rc = scratch_address(mod->dev, &ptr);
+    if (rc) return rc;
+    hipFunction_t f;
+    rc = find_function(mod, kernel, &f);
+    if (rc) return rc;
+    if (!stream) {
+        int cur = -1;
+        HIP_TRY(hipGetDevice(&cur));
+        if (cur != mod->dev)
+            return fail(UFO_E_INVALID, "default-stream launch on device %d of a module loaded on device %d", cur,
+                        mod->dev);
+    }
+    // clear the words the kernel may use, on the launch stream, before it
+    const unsigned int words = static_cast<unsigned int>((scratch_bytes + 3) / 4);
+    const unsigned int blocks = (words + 255) / 256 < 64 ? (words + 255) / 256 : 64;
+    hipLaunchKernelGGL(ufo_scratch_clear, dim3(blocks), dim3(256), 0, static_cast<hipStream_t>(stream),
+                       static_cast<uint32_t *>(ptr), words);
+    HIP_TRY(hipGetLastError());
+    // the caller's argument list (on the stack) with the scratch slot pointing at the device address
+    void *local[kMaxNoallocArgs];
+    for (size_t i = 0; i < nargs; ++i) local[i] = args[i];
+    local[scratch_arg] = &ptr;
+    HIP_TRY(hipModuleLaunchKernel(f, cfg->grid[0], cfg->grid[1], cfg->grid[2], cfg->block[0], cfg->block[1],
+                                  cfg->block[2], cfg->shared_bytes, static_cast<hipStream_t>(stream), local,
                                   nullptr));
     return UFO_OK;
 }

@@ -60,6 +60,9 @@ _SIGS = {
                                        ctypes.POINTER(ctypes.c_int)]),
     "ufo_launch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.POINTER(_Cfg),
                                   ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t, ctypes.c_void_p]),
+    "ufo_launch_noalloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.POINTER(_Cfg),
+                                          ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t, ctypes.c_size_t,
+                                          ctypes.c_size_t, ctypes.c_void_p]),
     "ufo_sync": (ctypes.c_int, [ctypes.c_void_p]),
     "ufo_malloc": (ctypes.c_int, [ctypes.c_size_t, ctypes.POINTER(ctypes.c_void_p)]),
     "ufo_free": (ctypes.c_int, [ctypes.c_void_p]),
@@ -269,6 +272,22 @@ class Module:
         cfg = self._cfg(grid, block, shared)
         check(lib().ufo_launch(self.handle, kernel.encode(), ctypes.byref(cfg), arr, len(holders),
                                ctypes.c_void_p(stream)), f"launch {kernel}")
+
+    def launch_noalloc(self, kernel: str, grid, block, args, scratch_arg: int, scratch_bytes: int,
+                       stream=None, shared=0):
+        """As ``launch``, with argument ``scratch_arg`` (a placeholder,
+        e.g. None) replaced by the library's own device scratch array, of
+        which the first ``scratch_bytes`` are zeroed before the launch; no
+        device memory is allocated.  Calls that overlap on one device share
+        that array and race."""
+        if stream is None:
+            stream = torch.cuda.current_stream(self.ctx.torch_device).cuda_stream
+        args = [ctypes.c_void_p(0) if i == scratch_arg or a is None else a for i, a in enumerate(args)]
+        arr, holders = self._args(args)
+        cfg = self._cfg(grid, block, shared)
+        check(lib().ufo_launch_noalloc(self.handle, kernel.encode(), ctypes.byref(cfg), arr, len(holders),
+                                       scratch_arg, scratch_bytes, ctypes.c_void_p(stream)),
+              f"launch {kernel} (no allocation)")
 
     def time(self, kernel: str, grid, block, args, reps=1, stream=None, shared=0) -> float:
         """Milliseconds of ``reps`` back-to-back launches, HIP events on the
