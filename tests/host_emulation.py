@@ -126,8 +126,9 @@ _KINDS = {
     "optics": ("ufloat *tracks = (ufloat *)malloc(sizeof(ufloat) * 3 * S * 6);", "%s(pool, tracks, S);",
                "free(tracks);"),
     "jvp": ("ufloat *out = (ufloat *)malloc(sizeof(ufloat) * n * 20);", "%s(pool, out, n);", "free(out);"),
-    "co": ("ufloat *orbits = (ufloat *)malloc(sizeof(ufloat) * n * 4); int *steps = (int *)malloc(sizeof(int) * n);",
-           "%s(pool, orbits, steps, n);", "free(orbits); free(steps);"),
+    "co": ("ufloat *orbits = (ufloat *)malloc(sizeof(ufloat) * n * 4); int *steps = (int *)malloc(sizeof(int) * n);"
+           " int *conv = (int *)malloc(sizeof(int) * n);",
+           "%s(pool, orbits, steps, conv, n);", "free(orbits); free(steps); free(conv);"),
 }
 
 _KIND_MAIN = r"""
@@ -168,3 +169,75 @@ def sanitized_run_kind(kernel, particles, columns, samples=0):
                     "-fno-sanitize-recover=all", "-w", "-o", exe, cpp], check=True)
     r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
     return r.returncode, r.stderr
+
+
+_ORBIT_DRIVERS = {
+    "co": r"""
+extern "C" void host_run(const ufloat *pool, ufloat *orbits, int *steps, int *conv, int n, int block)
+{
+    for (int b = 0; b < (n + block - 1) / block; ++b) {
+        blockIdx.x = b;
+        for (int t = 0; t < block; ++t) { threadIdx.x = t; %s(pool, orbits, steps, conv, n); }
+    }
+}
+""",
+    "co_seed": r"""
+extern "C" void host_run(const ufloat *table, const ufloat *guess, ufloat *orbits, int *steps, int *conv, int n,
+                         int block)
+{
+    for (int b = 0; b < (n + block - 1) / block; ++b) {
+        blockIdx.x = b;
+        for (int t = 0; t < block; ++t) { threadIdx.x = t; %s(table, guess, orbits, steps, conv, n); }
+    }
+}
+""",
+}
+
+
+def _compile_kind(kernel):
+    src = _PRELUDE + kernel.src + (_ORBIT_DRIVERS[kernel.kind] % kernel.name)
+    key = hashlib.sha1(src.encode()).hexdigest()[:20]
+    os.makedirs(_CACHE, exist_ok=True)
+    so = os.path.join(_CACHE, f"o_{key}.so")
+    if not os.path.exists(so):
+        cpp = os.path.join(_CACHE, f"o_{key}.cpp")
+        with open(cpp, "w") as f:
+            f.write(src)
+        tmp = so + f".{os.getpid()}"
+        subprocess.run(["g++", "-std=c++17", "-O2", "-ffp-contract=off", "-fno-fast-math", "-fPIC", "-shared",
+                        "-w", "-o", tmp, cpp], check=True)
+        os.replace(tmp, so)
+    return ctypes.CDLL(so)
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def run_orbits(kernel, pool):
+    """Host run of a generated ClosedOrbit kernel (``tangent.co_kernel``)
+    over the rows of ``pool``: (orbits [n, 4], steps [n], converged [n])."""
+    lib = _compile_kind(kernel)
+    dt = np.float32 if kernel.fp32 else np.float64
+    pool = np.ascontiguousarray(pool, dtype=dt)
+    n = pool.shape[0]
+    orbits = np.full((n, 4), np.nan, dtype=dt)
+    steps = np.full(n, -1, dtype=np.int32)
+    conv = np.full(n, -1, dtype=np.int32)
+    lib.host_run(_ptr(pool), _ptr(orbits), _ptr(steps), _ptr(conv), ctypes.c_int(n), ctypes.c_int(kernel.block))
+    return orbits, steps, conv != 0
+
+
+def run_seed_orbits(kernel, table, guess):
+    """Host run of a generated per-seed closed-orbit kernel
+    (``batch_adjoint.co_seed_kernel``): (orbits, steps, converged)."""
+    lib = _compile_kind(kernel)
+    table = np.ascontiguousarray(table, dtype=np.float64)
+    guess = np.ascontiguousarray(guess, dtype=np.float64)
+    n = guess.shape[0]
+    orbits = np.full((n, 4), np.nan)
+    steps = np.full(n, -1, dtype=np.int32)
+    conv = np.full(n, -1, dtype=np.int32)
+    lib.host_run(_ptr(table), _ptr(guess), _ptr(orbits), _ptr(steps), _ptr(conv), ctypes.c_int(n),
+                 ctypes.c_int(kernel.block))
+    return orbits, steps, conv != 0

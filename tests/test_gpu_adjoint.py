@@ -264,3 +264,56 @@ def test_gradient_thick_elements():
     err = np.abs(g - fd) / np.max(np.abs(fd))
     print("max rel err vs central differences:", err.max(), "|g|", np.abs(g).max())
     assert err.max() < 1e-6
+
+
+def test_backward_uses_the_buffers_of_its_own_forward():
+    """Two differentiable passes on one tracker (bunch A at theta, then
+    bunch B of the same size at another theta) before either backward: each
+    gradient equals that of its pass differentiated alone (the reverse pass
+    reads the checkpoints and coefficients of the forward that built its
+    graph, not the tracker's latest).  The same for BatchedAdjoint with
+    explicit saved buffers."""
+    cells, turns = 4, 32
+    ring, shared, theta, bunch_a = _fodo_setup(cells, 1024, seed=1)
+    bunch_b = lattices.gaussian_bunch(1024, seed=2, sx=3e-3, sy=3e-3)
+    flags = ufo.KICK | ufo.DOUBLE_PRECISION
+    tr = ufo.AdjointTracker(ring, flags, turns, COORDS, shared, every=8)
+    dev = tr.device
+
+    def grad(theta_, bunch_):
+        th = torch.tensor(theta_, device=dev, requires_grad=True)
+        x0 = torch.tensor(bunch_, device=dev, requires_grad=True)
+        xf, _ = adjoint.track(tr, th, x0)
+        return adjoint.amplitude_loss(xf), th, x0
+
+    la, tha, xa = grad(theta, bunch_a)
+    la.backward()
+    alone_a = (tha.grad.clone(), xa.grad.clone())
+    lb, thb, xb = grad(theta * 1.01, bunch_b)
+    lb.backward()
+    alone_b = (thb.grad.clone(), xb.grad.clone())
+    la, tha, xa = grad(theta, bunch_a)
+    lb, thb, xb = grad(theta * 1.01, bunch_b)
+    la.backward()                                 # the older graph first
+    lb.backward()
+    for (g_th, g_x), (a_th, a_x) in (((tha.grad, xa.grad), alone_a), ((thb.grad, xb.grad), alone_b)):
+        assert torch.allclose(g_th, a_th, rtol=1e-12, atol=0) and torch.allclose(g_x, a_x, rtol=1e-12, atol=0)
+    assert not torch.allclose(alone_a[0], alone_b[0], rtol=1e-3)
+
+    from ufo_amd.batch_adjoint import BatchedAdjoint
+    bring, prm, vals, counts, fam = lattices.dba_family_model(seeds=2, count_range=(100, 200))
+    ba = BatchedAdjoint(bring, flags, 16, COORDS, prm, vals, counts, fam[:2])
+    n = int(counts.sum())
+    rng = np.random.default_rng(3)
+    x0s = [np.stack([rng.normal(0, s, n), np.zeros(n), rng.normal(0, s / 2, n), np.zeros(n), np.zeros(n)], 1)
+           for s in (2e-3, 4e-3)]
+    lam = torch.ones((n, 4), dtype=torch.float64, device=dev)
+    alone = []
+    for x0 in x0s:
+        ba.forward(x0)
+        alone.append(ba.backward(lam)[1].clone())
+    _, _, saved_a = ba.forward_saved(x0s[0])
+    ba.forward(x0s[1])
+    ga = ba.backward(lam, saved_a)[1]
+    assert torch.allclose(ga, alone[0], rtol=1e-12, atol=0)
+    assert not torch.allclose(alone[0], alone[1], rtol=1e-3)

@@ -138,3 +138,62 @@ def test_emitted_optics_tangent_and_orbit_kernels_stay_in_bounds(kind, double):
         k = tangent.co_kernel(ring, fl, [(1, "dx"), (3, "dy")], 8, 1e-15)
     rc, report = host_emulation.sanitized_run_kind(k, n, cols, samples)
     assert rc == 0, report[-2000:]
+
+
+def _one_turn_residual(ring, flags, prm, pool, orbits):
+    """max |F(X) - X| of the oracle's one-turn map at dp = 0 for each row."""
+    full = np.concatenate([orbits, pool[:, 4:]], axis=1)
+    f = runner.track(ring, full, flags=flags | ufo.FIVED, turns=1, parameters=prm, where=[-1])[:, 0, :4]
+    return np.max(np.abs(f - orbits), axis=1)
+
+
+def test_closed_orbit_keeps_the_best_evaluated_iterate():
+    """The ClosedOrbit kernel built for the host: a converged orbit is a
+    fixed point of the oracle's one-turn map and flagged converged; a guess
+    lost in its first turn comes back unchanged and not converged; with one
+    Newton step allowed the guess (the only evaluated iterate) comes back,
+    never the unevaluated step; with tol = 0 (never reached) the returned
+    orbit's residual does not grow as the step cap rises."""
+    from ufo_amd import tangent
+    ring = lattices.fodo_ring(16)
+    fl = ufo.KICK | ufo.DOUBLE_PRECISION
+    prm = ["x", "px", "y", "py", (0, "dx"), (3, "dy")]   # QF and QD of the first cell
+    pool = np.array([[0.0, 0.0, 0.0, 0.0, 1e-4, -5e-5],
+                     [0.05, 0.0, 0.0, 0.0, 1e-4, 0.0],            # beyond the 40 mm aperture: lost in turn 1
+                     [2e-3, 1e-4, -1e-3, 2e-5, 1e-4, 5e-5]])
+    orb, steps, conv = host_emulation.run_orbits(tangent.co_kernel(ring, fl, prm, 64, 1e-15), pool)
+    assert conv.tolist() == [True, False, True]
+    assert np.array_equal(orb[1], pool[1, :4]) and steps[1] == 0
+    res = _one_turn_residual(ring, fl, prm, pool, orb)
+    assert res[0] < 1e-15 and res[2] < 1e-15 and np.max(np.abs(orb[0])) > 1e-6
+    one, steps1, conv1 = host_emulation.run_orbits(tangent.co_kernel(ring, fl, prm, 1, 1e-15), pool)
+    assert np.array_equal(one, pool[:, :4]) and not conv1.any() and steps1.tolist() == [1, 0, 1]
+    prev = None
+    for cap in range(1, 7):
+        o, s, c = host_emulation.run_orbits(tangent.co_kernel(ring, fl, prm, cap, 0.0), pool[2:])
+        r = _one_turn_residual(ring, fl, prm, pool[2:], o)[0]
+        assert not c[0] and s[0] == cap
+        if prev is not None:
+            assert r <= prev * (1 + 1e-6) + 1e-17, (cap, r, prev)
+        prev = r
+    assert prev < 1e-15
+
+
+def test_seed_closed_orbit_lost_seed_is_not_converged():
+    """The per-seed closed-orbit kernel built for the host: seeds from a
+    zero guess converge to oracle fixed points; a seed whose guess is lost in
+    its first turn is reported not converged and returned unchanged."""
+    from ufo_amd import batch
+    from ufo_amd.batch_adjoint import co_seed_kernel
+    ring, prm, vals, _, _ = lattices.dba_family_model(seeds=3)
+    fl = ufo.KICK | ufo.DOUBLE_PRECISION
+    k, layout = co_seed_kernel(ring, fl, prm, 32, 1e-13)
+    guess = np.zeros((3, 4))
+    guess[1, 0] = 0.05                                            # beyond the 30 mm aperture
+    orb, steps, conv = host_emulation.run_seed_orbits(k, batch.build_table(layout, vals), guess)
+    assert conv.tolist() == [True, False, True]
+    assert np.array_equal(orb[1], guess[1]) and steps[1] == 0
+    ok = [0, 2]
+    f = runner.track(ring, np.concatenate([orb[ok], vals[ok]], axis=1), flags=fl | ufo.FIVED, turns=1,
+                     parameters=["x", "px", "y", "py"] + prm, where=[-1])[:, 0, :4]
+    assert np.max(np.abs(f - orb[ok])) < 1e-13 and np.max(np.abs(orb[ok])) > 1e-7

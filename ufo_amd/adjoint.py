@@ -33,6 +33,7 @@ from typing import Dict, List
 import torch
 
 from . import codegen, ir
+from .lattice import lower
 from .codegen import Env, _fma, _fma_neg, is_zero, kick_coeffs
 from .flags import DOUBLE_PRECISION
 from .sim import Simulation
@@ -354,8 +355,7 @@ def emit_thick_reverse(op, env: Env, out: List[str], g: _Grad):
 # ---------------------------------------------------------------------------
 
 def _turn_ops(line, flags, parameters):
-    pieces = line.method(flags=flags, parameters=parameters)
-    return [op for piece in pieces for op in piece]
+    return lower(line, flags, parameters)
 
 
 def _reverse_layout(ops, env: Env, ntheta: int):
@@ -631,7 +631,17 @@ class AdjointTracker(Simulation):
         return th
 
     def forward(self, x0, theta):
-        """x0 [particles, len(coords)], theta [len(lattice_parameters)]."""
+        """x0 [particles, len(coords)], theta [len(lattice_parameters)].
+        The buffers the reverse pass needs are kept for the next
+        ``backward()`` call without ``saved``."""
+        xf, lost, self._saved = self.forward_saved(x0, theta)
+        return xf, lost
+
+    def forward_saved(self, x0, theta):
+        """As ``forward``, returning (final state, lost, saved) where
+        ``saved`` (pool, coefficient table, checkpoints, lost) belongs to this
+        pass only: give it to ``backward(lam, saved)`` to differentiate this
+        pass even after later forwards."""
         fwd = self.modules()[0]
         x0 = torch.as_tensor(x0, dtype=torch.float64, device=self.device).contiguous()
         if x0.dim() != 2 or x0.shape[1] != len(self.coords):
@@ -644,16 +654,18 @@ class AdjointTracker(Simulation):
         lost = torch.empty((n,), dtype=torch.int32, device=self.device)
         if n:
             fwd.launch(self.kfwd.name, (n + 255) // 256, 256, [pool, ct, ckpt, xf, lost, n])
-        self._saved = (pool, ct, ckpt, lost)
-        return xf, lost
+        return xf, lost, (pool, ct, ckpt, lost)
 
-    def backward(self, lam):
-        """lam = dL/d(final x, px, y, py) [particles, 4] for the last
-        ``forward``; returns (dL/dx0 [particles, 4], dL/dtheta)."""
+    def backward(self, lam, saved=None):
+        """lam = dL/d(final x, px, y, py) [particles, 4] for the pass whose
+        ``saved`` buffers are given (default: the last ``forward``); returns
+        (dL/dx0 [particles, 4], dL/dtheta)."""
         bwd = self.modules()[1]
-        if self._saved is None:
+        if saved is None:
+            saved = self._saved
+        if saved is None:
             raise RuntimeError("backward() needs a forward() first")
-        pool, th, ckpt, lost = self._saved
+        pool, th, ckpt, lost = saved
         n = pool.shape[0]
         lam = torch.nan_to_num(torch.as_tensor(lam, dtype=torch.float64, device=self.device), nan=0.0).contiguous()
         if tuple(lam.shape) != (n, 4):
@@ -673,8 +685,9 @@ class AdjointTracker(Simulation):
 class _TrackFunction(torch.autograd.Function):
     @staticmethod
     def forward(ctx, theta, x0, tracker):
-        xf, lost = tracker.forward(x0, theta)
+        xf, lost, saved = tracker.forward_saved(x0, theta)
         ctx.tracker = tracker
+        ctx.saved = saved          # this pass's own buffers, whatever the tracker runs next
         ctx.ncoord = x0.shape[1]
         ctx.coords = tracker.coords
         ctx.mark_non_differentiable(lost)
@@ -682,7 +695,7 @@ class _TrackFunction(torch.autograd.Function):
 
     @staticmethod
     def backward(ctx, grad_xf, grad_lost):
-        gx0, gth = ctx.tracker.backward(grad_xf)
+        gx0, gth = ctx.tracker.backward(grad_xf, ctx.saved)
         g0 = torch.zeros((gx0.shape[0], ctx.ncoord), dtype=gx0.dtype, device=gx0.device)
         for j, name in enumerate(ctx.coords):
             if name in ("x", "px", "y", "py"):

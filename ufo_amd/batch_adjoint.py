@@ -21,6 +21,7 @@ import numpy as np
 import torch
 
 from . import adjoint, batch, codegen, ir, tangent
+from .lattice import lower
 from .adjoint import ACCW
 from .flags import DOUBLE_PRECISION
 from .sim import Simulation
@@ -29,7 +30,7 @@ TILE = 64
 
 
 def _ops(line, flags, parameters):
-    return [op for piece in line.method(flags=flags, parameters=parameters) for op in piece]
+    return lower(line, flags, parameters)
 
 
 def _table_env(flags, src, ncoord, diff_cols=None):
@@ -60,7 +61,10 @@ def _fill(text, layout):
 
 
 def co_seed_kernel(line, flags, lattice_parameters, iterations=32, tol=1e-13, name="ufo_co_seed"):
-    """Newton closed orbit per seed (lane = seed), dp = 0 (ufo/co.py:175-193)."""
+    """Newton closed orbit per seed (lane = seed), dp = 0 (ufo/co.py:175-193).
+    ``conv[sd]`` is 1 only when the residual reached ``tol``; an orbit that
+    is lost or does not reach it within ``iterations`` steps is returned as
+    the evaluated iterate with the smallest residual."""
     coords: List[str] = []
     ops = _ops(line, flags, coords + list(lattice_parameters))
     src = codegen._coord_sources(coords, 0, 0.0)
@@ -75,7 +79,7 @@ def co_seed_kernel(line, flags, lattice_parameters, iterations=32, tol=1e-13, na
     code += f"""
 extern "C" __global__ void __launch_bounds__(64)
 {name}(const ufloat *__restrict__ table, const ufloat *__restrict__ guess, ufloat *__restrict__ orbits,
-       int *__restrict__ steps, int seeds)
+       int *__restrict__ steps, int *__restrict__ conv, int seeds)
 {{
     {codegen._decls(env)}
     ufloat tx[NT], tpx[NT], ty[NT], tpy[NT];
@@ -85,6 +89,8 @@ extern "C" __global__ void __launch_bounds__(64)
     z = {zero}; dp = {zero}; oodppo = {one};
 {chr(10).join('    ' + c for c in env.inv_code)}
     ufloat X[4] = {{guess[sd * 4], guess[sd * 4 + 1], guess[sd * 4 + 2], guess[sd * 4 + 3]}};
+    ufloat best = __builtin_huge_val(), XB[4] = {{X[0], X[1], X[2], X[3]}};
+    bool done = false;
     int it = 0;
     for (; it < {int(iterations)}; ++it) {{
         x = X[0]; px = X[1]; y = X[2]; py = X[3];
@@ -99,7 +105,9 @@ extern "C" __global__ void __launch_bounds__(64)
         }}
         ufloat r[4] = {{x - X[0], px - X[1], y - X[2], py - X[3]}};
         if (isnan(x) || isnan(px) || isnan(y) || isnan(py)) break;
-        if (fmax(fmax(fabs(r[0]), fabs(r[1])), fmax(fabs(r[2]), fabs(r[3]))) <= {env.lit(tol)}) break;
+        ufloat rmax = fmax(fmax(fabs(r[0]), fabs(r[1])), fmax(fabs(r[2]), fabs(r[3])));
+        if (rmax <= {env.lit(tol)}) {{ done = true; break; }}
+        if (rmax < best) {{ best = rmax; XB[0] = X[0]; XB[1] = X[1]; XB[2] = X[2]; XB[3] = X[3]; }}
         ufloat A[4][4];
 #pragma unroll
         for (int k = 0; k < NT; ++k) {{
@@ -111,9 +119,11 @@ extern "C" __global__ void __launch_bounds__(64)
 #pragma unroll
         for (int i = 0; i < 4; ++i) X[i] -= r[i];
     }}
+    if (!done) {{ X[0] = XB[0]; X[1] = XB[1]; X[2] = XB[2]; X[3] = XB[3]; }}
     ufloat *o = orbits + (size_t)sd * 4;
     o[0] = X[0]; o[1] = X[1]; o[2] = X[2]; o[3] = X[3];
     steps[sd] = it;
+    conv[sd] = done ? 1 : 0;
 }}
 """
     return codegen.Kernel(name=name, src=code, kind="co_seed", fp32=False, flops_per_turn=codegen._count(ops, env),
@@ -123,7 +133,10 @@ extern "C" __global__ void __launch_bounds__(64)
 class SeedClosedOrbit(Simulation):
     """Closed orbit of every seed (x, px, y, py at the ring start, dp = 0).
     ``seed_values`` [seeds, len(lattice_parameters)]; ``guess`` [seeds, 4]
-    (default 0).  After ``run()``: ``orbits`` [seeds, 4], ``steps``."""
+    (default 0).  After ``run()``: ``orbits`` [seeds, 4], ``steps`` (Newton
+    steps taken) and ``converged`` (the residual reached ``tol``; a seed whose
+    orbit is lost or stays above ``tol`` gets the evaluated iterate with the
+    smallest residual and False)."""
 
     def __init__(self, line, flags, lattice_parameters, seed_values, iterations=32, tol=1e-13, context=None,
                  options=None):
@@ -141,11 +154,13 @@ class SeedClosedOrbit(Simulation):
         guess = torch.tensor(self.guess, dtype=torch.float64, device=d)
         orbits = torch.empty((self.seeds, 4), dtype=torch.float64, device=d)
         steps = torch.empty((self.seeds,), dtype=torch.int32, device=d)
-        self.module().launch(self.kernel.name, (self.seeds + 63) // 64, 64, [tab, guess, orbits, steps, self.seeds])
+        conv = torch.empty((self.seeds,), dtype=torch.int32, device=d)
+        if self.seeds:
+            self.module().launch(self.kernel.name, (self.seeds + 63) // 64, 64,
+                                 [tab, guess, orbits, steps, conv, self.seeds])
         self.orbits = orbits.cpu().numpy()
         self.steps = steps.cpu().numpy()
-        # a seed that used every iteration did not reach the tolerance
-        self.converged = self.steps < self.iterations
+        self.converged = conv.cpu().numpy() != 0
         return self.orbits
 
 
@@ -400,7 +415,17 @@ class BatchedAdjoint(Simulation):
         return self._dev
 
     def forward(self, x0):
-        """x0 [particles, len(coords)], rows grouped by seed."""
+        """x0 [particles, len(coords)], rows grouped by seed.  The buffers
+        the reverse pass needs are kept for the next ``backward()`` call
+        without ``saved``."""
+        xf, lost, self._saved = self.forward_saved(x0)
+        return xf, lost
+
+    def forward_saved(self, x0):
+        """As ``forward``, returning (final state, lost, saved) where
+        ``saved`` (x0, checkpoints, lost) belongs to this pass only: give it
+        to ``backward(lam, saved)`` to differentiate this pass even after
+        later forwards."""
         b = self._buffers()
         x0 = torch.as_tensor(x0, dtype=torch.float64, device=self.device).contiguous()
         n = self.particles
@@ -413,16 +438,18 @@ class BatchedAdjoint(Simulation):
         if nt:
             self._mods[0].launch(self.kfwd.name, nt, 64,
                                  [x0, b["table"], b["ts"], b["t0"], b["tc"], ckpt, xf, lost, n])
-        self._saved = (x0, ckpt, lost)
-        return xf, lost
+        return xf, lost, (x0, ckpt, lost)
 
-    def backward(self, lam):
+    def backward(self, lam, saved=None):
         """lam = dL/d(final x, px, y, py) [particles, 4] (NaN rows of lost
-        particles count as 0) for the last ``forward``."""
+        particles count as 0) for the pass whose ``saved`` buffers are given
+        (default: the last ``forward``)."""
         b = self._buffers()
-        if self._saved is None:
+        if saved is None:
+            saved = self._saved
+        if saved is None:
             raise RuntimeError("backward() needs a forward() first")
-        x0, ckpt, lost = self._saved
+        x0, ckpt, lost = saved
         n = self.particles
         lam = torch.nan_to_num(torch.as_tensor(lam, dtype=torch.float64, device=self.device), nan=0.0).contiguous()
         if tuple(lam.shape) != (n, 4):

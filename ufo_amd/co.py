@@ -25,8 +25,9 @@ class ClosedOrbit(Simulation):
     ``tol`` is the stopping residual max |F(X) - X| in metres/radians (in
     FP32, Newton also stops once a step no longer lowers the residual); an
     orbit that does not reach it is returned as the iterate with the
-    smallest residual.
-    ``steps`` holds the Newton steps each particle took."""
+    smallest residual (the guess itself when its first turn is lost).
+    ``steps`` holds the Newton steps each particle took and ``converged``
+    whether its residual reached ``tol``."""
 
     def __init__(self, line, flags=0x00, particles=1000, parameters=(), dp=0., iterations=200,
                  context=None, options=None, tol=1e-15):
@@ -42,21 +43,29 @@ class ClosedOrbit(Simulation):
         self.orbits = self._orbits_host.numpy()
         self._steps_host = host_buffer((self.particles,), torch.int32)
         self.steps = self._steps_host.numpy()
+        self._conv_host = host_buffer((self.particles,), torch.int32)
         self._dev = None
+
+    @property
+    def converged(self):
+        """bool [particles]: the residual reached ``tol`` (after ``run()``)."""
+        return self._conv_host.numpy() != 0
 
     def run(self, threads=1):
         if self._dev is None:
             self._dev = (torch.empty(self._pool_host.shape, dtype=self.dtype, device=self.device),
                          torch.empty(self._orbits_host.shape, dtype=self.dtype, device=self.device),
-                         torch.empty(self._steps_host.shape, dtype=torch.int32, device=self.device))
-        pool, orbits, steps = self._dev
+                         torch.empty(self._steps_host.shape, dtype=torch.int32, device=self.device),
+                         torch.empty(self._conv_host.shape, dtype=torch.int32, device=self.device))
+        pool, orbits, steps, conv = self._dev
         pool.copy_(self._host_pool(), non_blocking=True)
         if self.particles:
             block = self.kernel.block
             grid = (self.particles + block - 1) // block
-            self.module().launch(self.kernel.name, grid, block, [pool, orbits, steps, self.particles])
+            self.module().launch(self.kernel.name, grid, block, [pool, orbits, steps, conv, self.particles])
         self._orbits_host.copy_(orbits, non_blocking=True)
         self._steps_host.copy_(steps, non_blocking=True)
+        self._conv_host.copy_(conv, non_blocking=True)
         self.stream().synchronize()
 
 

@@ -23,6 +23,7 @@ from __future__ import annotations
 from typing import List
 
 from . import codegen, ir
+from .lattice import lower
 from .codegen import Env, _fma, _fma_neg, is_zero, emit_op, kick_coeffs
 
 def _loop(body: str) -> str:
@@ -183,8 +184,7 @@ def jvp_kernel(line, flags, turns, parameters, dp, block=256, name="ufo_jvp"):
     turns; a particle stops after the turn at whose end x is NaN.  Output
     [particles][20]: x, px, y, py, then dx_k, dpx_k, dy_k, dpy_k for k = 0..3
     (column k is the derivative with respect to initial coordinate k)."""
-    pieces = line.method(flags=flags, parameters=parameters)
-    ops = [op for piece in pieces for op in piece]
+    ops = lower(line, flags, parameters)
     src = codegen._coord_sources(parameters, flags, dp)
     env = codegen._env_for(flags, src, ops)
     body, period = codegen._layout(ops, env, emit=emit_with_tangent)
@@ -232,10 +232,11 @@ def co_kernel(line, flags, parameters, iterations, tol, block=64, name="ufo_co")
     the particle's x, px, y, py parameters (0 when not named), as the
     reference seeds its simplex (ufo/co.py:208-213).  Output [particles][4];
     an orbit that is lost or does not reach ``tol`` within ``iterations``
-    steps is returned as the iterate with the smallest residual.  In FP32 the
+    steps is returned as the evaluated iterate with the smallest residual
+    (the guess itself if the first evaluation is lost), with ``converged[p]``
+    = 0; ``converged[p]`` = 1 when the residual reached ``tol``.  In FP32 the
     iteration also ends at the first step that does not lower the residual."""
-    pieces = line.method(flags=flags, parameters=parameters)
-    ops = [op for piece in pieces for op in piece]
+    ops = lower(line, flags, parameters)
     src = codegen._coord_sources(parameters, 0, 0.0)
     src["z"] = ("c", 0.0)
     src["dp"] = ("c", 0.0)
@@ -257,7 +258,8 @@ def co_kernel(line, flags, parameters, iterations, tol, block=64, name="ufo_co")
     code += "#define NT 4\n" + _SOLVE4
     code += f"""
 extern "C" __global__ void __launch_bounds__({block})
-{name}(const ufloat *__restrict__ pool, ufloat *__restrict__ orbits, int *__restrict__ steps, int particles)
+{name}(const ufloat *__restrict__ pool, ufloat *__restrict__ orbits, int *__restrict__ steps,
+       int *__restrict__ converged, int particles)
 {{
     {codegen._decls(env)}
     ufloat tx[NT], tpx[NT], ty[NT], tpy[NT];
@@ -299,6 +301,7 @@ extern "C" __global__ void __launch_bounds__({block})
     ufloat *o = orbits + (size_t)idx * 4;
     o[0] = X[0]; o[1] = X[1]; o[2] = X[2]; o[3] = X[3];
     steps[idx] = it;
+    converged[idx] = conv ? 1 : 0;
 }}
 """
     return codegen.Kernel(name=name, src=code, kind="co", fp32=env.fp32,
