@@ -4,12 +4,12 @@ The checker is the oracle's strict-IEEE build of the restated reference
 kernels.  Rounding noise is amplified by the dynamics, so the bar is set per
 particle from its measured sensitivity:
 
-  sensitivity  max of two differences against the strict oracle, both of
+  sensitivity  max of three differences against the strict oracle, all of
                them the reference's own arithmetic: its fast-math build
-               (stand-in for -cl-fast-relaxed-math -cl-mad-enable) and a
-               strict run with every input column moved by one ulp (all
-               coordinates: in uncoupled linear motion a change of x never
-               reaches the y plane).
+               (stand-in for -cl-fast-relaxed-math -cl-mad-enable) and two
+               strict runs with every input column moved by one ulp, up and
+               down (all coordinates: in uncoupled linear motion a change of
+               x never reaches the y plane).
   regular      same lost/alive flags in those runs and sensitivity <= 3e-13:
                rounding is not amplified beyond 3e-13 over the run.
 
@@ -19,7 +19,17 @@ the strict oracle with identical
 lost/alive flags; over all particles with stable flags, the median and 90th
 percentile of its error are no larger than twice those of the reference's
 own sensitivity (plus 1e-14), i.e. HIP is as close to the reference as the
-reference is to itself.
+reference is to itself; and on every one of those particles (the tail) its
+error is at most 4 times that particle's own sensitivity plus 1e-14, so a
+defect on a few particles cannot hide behind the aggregates.  Two rounding
+perturbations of one chaotic orbit grow in directions that differ at random,
+so the ratio of two such errors is heavy-tailed; with the sensitivity taken
+as the largest of three runs the worst ratio over the nonlinear workloads is
+below 4 (DBA ring at 20 mm, 64 turns: 3.8; FODO-16, 4096 particles, 1000
+turns: 0.13).  Where every particle is chaotic (the reference's own spread
+near 1e-6 after 30 turns through a strong octupole) a fourth independent
+run exceeds four times the largest of three on about one particle in a
+hundred, and the factor there is 8.
 """
 import numpy as np
 
@@ -27,6 +37,7 @@ REL_TOL = 1e-12
 REGULAR_TOL = 3e-13
 STAT_FACTOR = 2.0
 STAT_FLOOR = 1e-14
+TAIL_FACTOR = 4.0
 
 
 BOUND = 1.0    # metres / radians: beyond it a coordinate has left any aperture
@@ -63,14 +74,15 @@ def nan_flags(t):
 
 
 def reference_runs(track_fn, pool, fp32=False):
-    """strict, fast and one-ulp-perturbed strict runs of the oracle;
+    """strict and fast runs of the oracle and the list of its two
+    one-ulp-perturbed strict runs (every input moved up, then down);
     ``track_fn(pool, mode)`` runs it.  The ulp is that of the run's
     precision (an FP64 ulp vanishes when an FP32 run rounds its inputs)."""
     strict = track_fn(pool, "strict")
     fast = track_fn(pool, "fast")
     dt = np.float32 if fp32 else np.float64
-    moved = np.nextafter(np.array(pool, dtype=dt), dt(np.inf))
-    perturbed = track_fn(moved, "strict")
+    a = np.array(pool, dtype=dt)
+    perturbed = [track_fn(np.nextafter(a, dt(np.inf)), "strict"), track_fn(np.nextafter(a, dt(-np.inf)), "strict")]
     return strict, fast, perturbed
 
 
@@ -102,15 +114,24 @@ def loss_turn_runs(sa_fn, pool, fp32=False, track_fn=None):
 
 
 def compare_tracks(gpu, strict, fast, perturbed):
+    """``perturbed``: one perturbed oracle run or a list of them."""
+    runs = [fast] + (list(perturbed) if isinstance(perturbed, (list, tuple)) else [perturbed])
     ns = nan_flags(strict)
-    stable = np.all(ns == nan_flags(fast), axis=1) & np.all(ns == nan_flags(perturbed), axis=1)
-    sens = np.maximum(particle_err(fast, strict), particle_err(perturbed, strict))
+    stable = np.ones(len(ns), dtype=bool)
+    sens = np.zeros(len(ns))
+    for run in runs:
+        stable &= np.all(ns == nan_flags(run), axis=1)
+        sens = np.maximum(sens, particle_err(run, strict))
     e_gpu = particle_err(gpu, strict)
     regular = stable & (sens <= REGULAR_TOL)
     flags_ok = np.array_equal(nan_flags(gpu)[stable], ns[stable])
+    tail = e_gpu[stable] / (sens[stable] + STAT_FLOOR / TAIL_FACTOR)
     r = {
         "regular": regular,
         "flags_ok": flags_ok,
+        "max_err": float(np.max(e_gpu[stable])) if np.any(stable) else 0.0,
+        "tail_ratio": float(np.max(tail)) if tail.size else 0.0,
+        "tail_over": int(np.sum(tail > TAIL_FACTOR)),
         "worst_regular": float(np.max(e_gpu[regular])) if np.any(regular) else 0.0,
         "median_gpu": float(np.median(e_gpu[stable])) if np.any(stable) else 0.0,
         "median_ref": float(np.median(sens[stable])) if np.any(stable) else 0.0,
@@ -123,12 +144,16 @@ def compare_tracks(gpu, strict, fast, perturbed):
 def summary(r):
     return (f"regular {int(r['regular'].sum())}/{len(r['regular'])}, worst on regular {r['worst_regular']:.2e}; "
             f"median err hip {r['median_gpu']:.2e} vs reference self {r['median_ref']:.2e}; "
-            f"p90 hip {r['p90_gpu']:.2e} vs {r['p90_ref']:.2e}")
+            f"p90 hip {r['p90_gpu']:.2e} vs {r['p90_ref']:.2e}; max err hip {r['max_err']:.2e}, "
+            f"worst err/sensitivity {r['tail_ratio']:.2f} ({r['tail_over']} over {TAIL_FACTOR:g})")
 
 
-def assert_parity(r, min_regular_frac=0.25):
+def assert_parity(r, min_regular_frac=0.25, tail_factor=TAIL_FACTOR):
+    """The bar (module docstring).  ``tail_factor`` bounds each particle's
+    error by that multiple of its own sensitivity (plus 1e-14)."""
     assert r["flags_ok"], "lost/alive flags differ on particles whose flags are stable"
     assert r["worst_regular"] <= REL_TOL, summary(r)
+    assert r["tail_ratio"] <= tail_factor, summary(r)
     assert r["median_gpu"] <= STAT_FACTOR * r["median_ref"] + STAT_FLOOR, summary(r)
     assert r["p90_gpu"] <= STAT_FACTOR * r["p90_ref"] + STAT_FLOOR, summary(r)
     assert r["regular"].mean() >= min_regular_frac, summary(r)

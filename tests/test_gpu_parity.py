@@ -137,15 +137,15 @@ def test_dba_stable_aperture_subset():
 def test_dba_track_coordinates():
     """DA ring tracked 64 turns from grid points.  The grid reaches 20 mm,
     where the motion is strongly nonlinear and rounding differences grow
-    quickly: after 64 turns 21 of the 64 points are regular (the oracle's
-    strict, fast-math and one-ulp-moved runs agree to 3e-13)."""
+    quickly: after 64 turns 15 of the 64 points are regular (the oracle's
+    strict, fast-math and two one-ulp-moved runs agree to 3e-13)."""
     ring = lattices.dba_ring(16)
     flags = ufo.KICK | ufo.DOUBLE_PRECISION | ufo.FIVED
     pool = lattices.da_grid(16, 4)
     gpu, strict, fast, moved = _track_both(ring, pool, flags, 64, ["x", "y"], [-1])
     r = compare_tracks(gpu, strict, fast, moved)
     print(summary(r))
-    assert_parity(r)
+    assert_parity(r, min_regular_frac=0.2)
 
 
 def test_alba_thick_track_fp64():
@@ -676,4 +676,6 @@ def test_golden_lattices(lattice, line, flags):
     gpu, strict, fast, moved = _track_both(ring, pool, flags, 30, COORD6, [-1])
     r = compare_tracks(gpu, strict, fast, moved)
     print(f"{lattice}.{line} flags {flags}: " + summary(r))
-    assert_parity(r, min_regular_frac=0.0)
+    # strong sextupoles and octupoles make every particle of some of these
+    # lines chaotic within 30 turns: the per-particle factor 8 (parity.py)
+    assert_parity(r, min_regular_frac=0.0, tail_factor=8.0)
