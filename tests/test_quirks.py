@@ -113,3 +113,34 @@ def test_duplicate_observation_point_loses_the_particle_as_in_the_reference():
     assert np.all(np.isnan(host[:, 1, 1])) and np.all(np.isnan(ref[:, 1, 1]))
     assert np.all(np.isnan(host[:, 2:, :4])) and np.all(np.isnan(ref[:, 2:, :4]))
     np.testing.assert_allclose(host[:, 0], ref[:, 0], rtol=1e-12, atol=1e-18)
+
+
+def test_indexed_strength_attributes():
+    """``dkN``/``dkNs`` read and write single entries of an element's error
+    lists and ``kN``/``kNs`` those of a Multipole's strength lists
+    (ufo/elements.py:37-50,128-145); ``dkN`` keyword overrides of method()
+    extend the error list with zeros to reach N (ufo/elements.py:14-23),
+    ``kN`` overrides of a Multipole need the entry to exist."""
+    q = ufo.Quadrupole("Q", length=0.5, k1=1.0, dkn=[0.0, 0.0, 3.0], dks=[0.0, 1.5])
+    assert (q.dk2, q.dk1s) == (3.0, 1.5)
+    q.dk2 = 4.0
+    q.dk0s = -2.0
+    assert q.dkn == [0.0, 0.0, 4.0] and q.dks == [-2.0, 1.5]
+    for missing in ("dk7", "dk5s", "dkx", "dk"):
+        with pytest.raises(AttributeError):
+            getattr(q, missing)
+    m = ufo.Multipole("M", knl=[0.0, 0.5, 10.0], ksl=[0.0, 0.25])
+    assert (m.k1, m.k2, m.k1s) == (0.5, 10.0, 0.25)
+    m.k2 = 12.0
+    m.k0s = 0.125
+    assert m.knl == [0.0, 0.5, 12.0] and m.ksl == [0.125, 0.25]
+    # a dkN override beyond the list pads with zeros; the same strength given
+    # through dkn= lowers to the same ops
+    grown = ufo.Sextupole("S", length=0.2, k2=30.0).method(flags=FL, dk4=7.0, dk3s=2.0)
+    given = ufo.Sextupole("S", length=0.2, k2=30.0).method(flags=FL, dkn=[0.0, 0.0, 0.0, 0.0, 7.0],
+                                                           dks=[0.0, 0.0, 0.0, 2.0])
+    assert grown == given
+    assert ufo.Multipole("M", knl=[0.0, 0.5]).method(flags=FL, k1=0.75) == \
+        ufo.Multipole("M", knl=[0.0, 0.75]).method(flags=FL)
+    with pytest.raises(IndexError):
+        ufo.Multipole("M", knl=[0.0, 0.5]).method(flags=FL, k3=1.0)

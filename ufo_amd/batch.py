@@ -36,6 +36,7 @@ import numpy as np
 import torch
 
 from . import codegen, ir, runtime
+from .lattice import lower
 from .sim import Simulation, host_buffer
 
 TILE = 64          # particles per tile and lane-particle (a tile is one wave's work)
@@ -43,7 +44,7 @@ PAIR_RE = re.compile(r"\b(x|px|y|py|t\d+)\b")
 
 
 def _turn_ops(line, flags, parameters):
-    return [op for piece in line.method(flags=flags, parameters=parameters) for op in piece]
+    return lower(line, flags, parameters)
 
 
 def _paired(lines):

@@ -37,6 +37,7 @@ from typing import Dict, List, Optional
 import numpy as np
 
 from . import crmath, ir
+from .lattice import lower
 from .flags import FIVED, DOUBLE_PRECISION
 from .ir import Expr, fold, is_zero
 
@@ -987,8 +988,7 @@ def sa_kernel(line, flags, turns, parameters, dp, block=256, name="ufo_sa") -> K
     """StableAperture: lost[p] = number of turns completed before the
     end-of-turn test |x| > 1 || |y| > 1 || isnan(x) fired, or ``turns``
     (ufo/sa.py:116-121)."""
-    pieces = line.method(flags=flags, parameters=parameters)
-    ops = [op for piece in pieces for op in piece]
+    ops = lower(line, flags, parameters)
     src = _coord_sources(parameters, flags, dp)
     env = _env_for(flags, src, ops)
     body, period = _layout(ops, env)
@@ -1054,12 +1054,7 @@ def track_kernel(line, flags, turns, parameters, where, dp, block=256, name="ufo
     state.  Splitting the turns over launches changes no result, and lets the
     caller copy finished samples back while later turns run."""
     where = list(where) if isinstance(where, (list, tuple)) else [where]
-    pieces = line.method(fracture=where, flags=flags, parameters=parameters)
-    ops = []
-    for i, piece in enumerate(pieces):
-        if i:
-            ops.append(ir.Observe())
-        ops += piece
+    ops = lower(line, flags, parameters, where)
     if -1 in where:
         ops.append(ir.Observe())
     dumps = sum(1 for op in ops if isinstance(op, ir.Observe))
@@ -1108,12 +1103,7 @@ def optics_kernel(line, flags, where, parameters, name="ufo_optics") -> Kernel:
     'dp' parameter if named) — dumped at the observation points
     (ufo/optics.py:87-115).  Output [3][count][6]; only x, px, y, py are
     written."""
-    pieces = line.method(fracture=where, flags=flags, parameters=parameters)
-    ops = []
-    for i, piece in enumerate(pieces):
-        if i:
-            ops.append(ir.Observe())
-        ops += piece
+    ops = lower(line, flags, parameters, where)
     if -1 in where:
         ops.append(ir.Observe())
     dumps = sum(1 for op in ops if isinstance(op, ir.Observe))

@@ -245,6 +245,11 @@ def const(v) -> Expr:
     return Expr(("c", float(v)))
 
 
+def param(i) -> Expr:
+    """Per-particle parameter column ``i`` (a row of the kernel's pool)."""
+    return Expr(("p", int(i)))
+
+
 def var(name) -> Expr:
     return Expr(("v", name))
 
