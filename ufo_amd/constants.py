@@ -2,7 +2,7 @@
 ufo/constants.py:9-14)."""
 
 speed_of_light = 299792458.0              # m/s
-electron_mass = 0.51099895000e6           # eV
+electron_mass = 0.51099895 * 1e6          # eV; this product (510998.94999999995) is the reference's value
 proton_mass = 938.27208816e6              # eV
 elementary_charge = 1.602176634e-19       # C
 vacuum_permittivity = 8.8541878128e-12    # F/m
