@@ -15,8 +15,9 @@ Also reported:
              launch time from HIP events on the launch stream, against the
              MI355X FP64 peak of 78.6 TFLOP/s;
   cpu_baseline  the oracle (the reference's kernels restated in C, gcc
-             -O3 -ffast-math, OpenMP over all host threads) on an evenly
-             strided sample of the same grid, as PET/s.
+             -O3 -march=native fast-math and strict-IEEE builds, OpenMP over
+             the host threads) on an evenly strided sample of the same
+             grid, as PET/s, with the host's CPU model and core counts.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
 """
@@ -50,6 +51,7 @@ def main():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-seed-adjoint", action="store_true")
     ap.add_argument("--no-fodo", action="store_true")
+    ap.add_argument("--no-lossy", action="store_true")
     ap.add_argument("--no-co-optics", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="MiB copied by one device-to-device copy before the timed region "
@@ -142,6 +144,9 @@ def main():
             "unit": "TFLOP/s",
             "frac": achieved_tflops / PEAK_FP64_TFLOPS,
             "traffic": measured_traffic(args),
+            "traffic_source": ("profiles/ufo_sa_dba_da_scan_1M_1024turns.md: rocprofv3 --pmc FETCH_SIZE and "
+                               "WRITE_SIZE of this workload, scaled by the counters of a 512 MiB device copy "
+                               "(committed profile, not measured in this run)"),
             "kernel": sa.kernel.name,
             "kernel_ms": float(np.mean(kernel_ms)),
             "peak_from_device": sa.ctx.device.fp64_peak / 1e12,
@@ -149,6 +154,9 @@ def main():
         "cpu_baseline": None,
     }
     _progress(f"DA scan: {float(np.mean(kernel_ms)):.1f} ms per launch")
+    if not args.no_lossy:
+        out["lossy_da"] = lossy_scan_timing(args.nx, args.ny, args.turns, args.steps, achieved_tflops)
+        _progress("lossy DA scan done")
     if not args.no_fodo:
         out["fodo_track"] = fodo_track_timing(cpu=not args.no_cpu)
         _progress("FODO Track done")
@@ -168,6 +176,52 @@ def main():
         out["cpu_baseline"] = cpu_baseline(ring, flags, pool, args.turns, E, args.cpu_seconds, gpu_lost=lost)
         _progress("CPU baseline done")
     print(json.dumps(out))
+
+
+def lossy_scan_timing(nx, ny, turns, reps, lossless_tflops, sext_scale=4.0):
+    """The DA scan with particle loss: the same ring, grid and turns with
+    the sextupole strengths scaled by 4 (k2 = +-152 m^-3), so that the
+    dynamic aperture falls inside the grid and about 40 % of the particles
+    are lost, most within 50 turns.  Lost lanes refill from the work
+    counter; the rate is the canonical FP64 work of the executed
+    particle-turns.  ``utilisation_vs_lossless`` is the achieved FP64 rate
+    over that of the lossless scan (whose issued and canonical FLOPs are
+    equal): the fraction of the FP64 issue the lossy launch turns into
+    useful particle-turns (loss, refill and the tail of the launch)."""
+    import numpy as np
+    import torch
+    import ufo_amd as ufo
+    from ufo_amd import lattices
+    ring = lattices.dba_ring(16, sext_scale=sext_scale)
+    flags = ufo.KICK | ufo.DOUBLE_PRECISION | ufo.FIVED
+    pool = lattices.da_grid(nx, ny)
+    sa = ufo.StableAperture(ring, flags=flags, turns=turns, particles=len(pool), parameters=["x", "y"])
+    sa.parameters[:] = pool
+    sa.module()
+    sa.pool_device.copy_(sa._pool_host)
+    sa.launch()
+    stream = torch.cuda.current_stream()
+    ms = []
+    for _ in range(max(1, reps)):
+        e = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+        e[0].record(stream)
+        sa.launch()
+        e[1].record(stream)
+        torch.cuda.synchronize()
+        ms.append(e[0].elapsed_time(e[1]))
+    lost = sa.lost_device.cpu().numpy().view(np.uint32)
+    executed = np.minimum(lost.astype(np.int64) + 1, turns)
+    pt = int(executed.sum())
+    k_s = float(np.mean(ms)) / 1e3
+    tflops = pt * sa.kernel.flops_per_turn / k_s / 1e12
+    return {"workload": f"DA scan with loss: 16-period DBA ring, sextupoles x{sext_scale:g}, {nx}x{ny} grid "
+                        f"x in [-20,20] mm, y in [0,8] mm, {turns} turns",
+            "kernel_ms": round(float(np.mean(ms)), 3), "pet_per_s": pt * sa.kernel.elements / k_s,
+            "lost_fraction": float(np.mean(lost < turns)),
+            "executed_particle_turns": pt, "executed_fraction": pt / (len(pool) * turns),
+            "loss_turn_quantiles_10_50_90": [float(q) for q in np.percentile(lost[lost < turns], [10, 50, 90])],
+            "tflops": tflops, "frac_fp64_peak": tflops / PEAK_FP64_TFLOPS,
+            "utilisation_vs_lossless": tflops / lossless_tflops, "lanes": sa.lanes}
 
 
 def fodo_track_timing(particles=4096, turns=1000, reps=3, cpu=True):
@@ -379,6 +433,7 @@ def seed_adjoint_timing(seeds=64, per_seed=8192, turns=1024, every=32):
             fw.append(ev[0].elapsed_time(ev[1]))
             bw.append(ev[2].elapsed_time(ev[3]))
     f_ms, b_ms = float(np.median(fw)), float(np.median(bw))
+    check = seed_gradient_check(ba, ring, flags, turns, prm, vals, fam, x0, seg, seeds, per_seed)
     return {"workload": f"{seeds} seeds x {per_seed} particles, 4-period DBA with errors, 6D, {turns} turns, "
                         f"gradient w.r.t. {len(fam)} sextupole families per seed",
             "closed_orbit_ms": round(co_ms, 2), "newton_steps_max": int(co.steps.max()),
@@ -386,7 +441,64 @@ def seed_adjoint_timing(seeds=64, per_seed=8192, turns=1024, every=32):
             "forward_ms": round(f_ms, 2), "reverse_ms": round(b_ms, 2),
             "gradient_over_forward": round((f_ms + b_ms) / f_ms, 3),
             "survivors": int((lost == turns).sum().item()), "particles": n,
-            "hiprtc_compile_s": round(compile_s, 1), "gradient_finite": bool(torch.isfinite(gseed).all().item())}
+            "hiprtc_compile_s": round(compile_s, 1), "gradient_finite": bool(torch.isfinite(gseed).all().item()),
+            **check}
+
+
+def seed_gradient_check(ba, ring, flags, turns, prm, vals, fam, x0, seg, seeds, per_seed, amp_max=30.0, h=1e-3):
+    """Per-seed gradients of every family against 4th-order central
+    differences (steps +-h, +-2h on the family strength of every seed at
+    once: seeds are independent).  The objective is the per-seed amplitude
+    loss over the bunch's regular core, the survivors whose final state
+    amplifies a change of the initial one by at most ``amp_max`` (from the
+    adjoint's own dL/dx0): a chaotic particle's final state is no smooth
+    function of the strengths at any usable step.  Reported: the largest
+    |adjoint - fd| / max(|fd|, 1e-3 max_family |fd|) over seeds and
+    families."""
+    import numpy as np
+    import torch
+    from ufo_amd import batch
+    from ufo_amd.batch_adjoint import BatchedAdjoint
+    dev = ba.device
+    xf0, _ = ba.forward(x0)
+    alive = torch.isfinite(xf0[:, 0])
+    lam = torch.zeros_like(xf0)
+    lam[:, 0] = 2 * torch.nan_to_num(xf0[:, 0])
+    lam[:, 2] = 2 * torch.nan_to_num(xf0[:, 2])
+    gx0, _ = ba.backward(lam)
+    mask = alive & (gx0.norm(dim=1) / lam.norm(dim=1).clamp_min(1e-300) <= amp_max)
+
+    def per_seed_loss(xf):
+        r2 = torch.where(mask, xf[:, 0] ** 2 + xf[:, 2] ** 2, torch.zeros_like(xf[:, 0]))
+        soft = torch.nn.functional.softplus(20.0 * (r2 / 0.03 ** 2 - 0.25)) / 20.0
+        pen = torch.where(mask, soft, torch.zeros_like(r2))
+        return torch.zeros(seeds, dtype=torch.float64, device=dev).index_add_(0, seg, r2 + 1e-5 * pen) / per_seed
+
+    xf_t = xf0.clone().requires_grad_(True)
+    per_seed_loss(xf_t).sum().backward()
+    g = ba.backward(torch.nan_to_num(xf_t.grad, nan=0.0))[1].cpu().numpy()
+    probe = BatchedAdjoint(ring, flags, turns, ["x", "px", "y", "py", "dp"], prm, vals, np.full(seeds, per_seed),
+                           fam)
+    probe._buffers()
+
+    def loss_at(v):
+        probe._dev["table"] = torch.tensor(batch.build_table(probe.layout, v), dtype=torch.float64, device=dev)
+        return per_seed_loss(probe.forward(x0)[0]).cpu().numpy()
+
+    fd = np.zeros_like(g)
+    for j, f in enumerate(fam):
+        c = prm.index(f)
+        for k, w in ((1, 8.0), (2, -1.0)):
+            vp, vm = vals.copy(), vals.copy()
+            vp[:, c] += k * h
+            vm[:, c] -= k * h
+            fd[:, j] += w * (loss_at(vp) - loss_at(vm))
+        fd[:, j] /= 12.0 * h
+    crit = np.abs(g - fd) / np.maximum(np.abs(fd), 1e-3 * np.max(np.abs(fd), axis=0))
+    return {"gradient_fd_max_rel": float(crit.max()),
+            "gradient_fd_check": f"{seeds} seeds x {len(fam)} families vs 4th-order central differences "
+                                 f"(h = {h:g}) of the loss over the regular core ({int(mask.sum())} of "
+                                 f"{int(alive.sum())} survivors, amplification <= {amp_max:g})"}
 
 
 def adjoint_timing(cells=48, particles=65536, turns=512, every=32, warmup=2, reps=5):
@@ -463,45 +575,74 @@ def measured_traffic(args):
     return None
 
 
-def _host_cpu():
+def _host_record():
+    """The host the CPU baseline ran on: lscpu's model name (else
+    /proc/cpuinfo's), logical CPUs (nproc), CPUs in this process's affinity
+    mask, and physical cores when lscpu reports them."""
+    import subprocess
+    rec = {"nproc": os.cpu_count(), "affinity_cpus": len(os.sched_getaffinity(0)), "model": None,
+           "physical_cores": None}
     try:
-        with open("/proc/cpuinfo") as f:
-            for line in f:
-                if line.startswith("model name"):
-                    return line.split(":", 1)[1].strip()
-    except OSError:
+        text = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        fields = dict(line.split(":", 1) for line in text.splitlines() if ":" in line)
+        fields = {k.strip(): v.strip() for k, v in fields.items()}
+        rec["model"] = fields.get("Model name")
+        cores, sockets = fields.get("Core(s) per socket"), fields.get("Socket(s)")
+        if cores and sockets and cores.isdigit() and sockets.isdigit():
+            rec["physical_cores"] = int(cores) * int(sockets)
+    except (OSError, subprocess.SubprocessError, ValueError):
         pass
-    return None
+    if not rec["model"]:
+        try:
+            with open("/proc/cpuinfo") as f:
+                for line in f:
+                    if line.startswith("model name"):
+                        rec["model"] = line.split(":", 1)[1].strip()
+                        break
+        except OSError:
+            pass
+    return rec
 
 
 def cpu_baseline(ring, flags, pool, turns, E, budget_s, gpu_lost=None):
-    """The oracle (restated reference kernels, gcc fast-math build, OpenMP)
-    on an evenly strided sample of the grid, sized to ~budget_s seconds.
-    With ``gpu_lost`` (the GPU's loss turns over the whole grid) it also
-    reports on how many sample points the two loss turns are identical."""
+    """The oracle (restated reference kernels, OpenMP over the host threads)
+    on an evenly strided sample of the grid sized to ~budget_s seconds, in
+    two builds: gcc -O3 -march=native fast-math (the analogue of the
+    reference's default -cl-fast-relaxed-math -cl-mad-enable, NaN tests
+    kept; ``value``) and strict IEEE (-ffp-contract=off, on a sample sized
+    to half the budget; ``strict``).  With ``gpu_lost`` (the GPU's loss
+    turns over the whole grid) it also reports on how many sample points
+    the fast build's loss turns are identical to the GPU's."""
     import numpy as np
     from oracle import runner
-    threads = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
-    probe = pool[:: max(1, len(pool) // 256)][:256]
-    t = time.perf_counter()
-    runner.stable_aperture(ring, probe, flags=flags, turns=turns, parameters=["x", "y"], mode="fast")
-    t = time.perf_counter()
-    lost = runner.stable_aperture(ring, probe, flags=flags, turns=turns, parameters=["x", "y"], mode="fast")
-    dt = time.perf_counter() - t
-    per_particle = dt / len(probe)
-    m = int(min(len(pool), max(256, budget_s / max(per_particle, 1e-9))))
-    stride = max(1, len(pool) // m)
-    sample = pool[::stride][:m]
-    t = time.perf_counter()
-    lost = runner.stable_aperture(ring, sample, flags=flags, turns=turns, parameters=["x", "y"], mode="fast")
-    dt = time.perf_counter() - t
-    pt = int(np.minimum(lost.astype(np.int64) + 1, turns).sum())
-    out = {"value": pt * E / dt, "unit": "particle-element-turns/s", "cores": threads, "kind": "port",
-           "host_cpu": _host_cpu(),
+    host = _host_record()
+    threads = int(os.environ.get("OMP_NUM_THREADS") or host["affinity_cpus"] or 1)
+    res = {}
+    for mode, budget in (("bench_fast", budget_s), ("bench_strict", budget_s / 2)):
+        probe = pool[:: max(1, len(pool) // 256)][:256]
+        runner.stable_aperture(ring, probe, flags=flags, turns=turns, parameters=["x", "y"], mode=mode)
+        t = time.perf_counter()
+        runner.stable_aperture(ring, probe, flags=flags, turns=turns, parameters=["x", "y"], mode=mode)
+        per_particle = (time.perf_counter() - t) / len(probe)
+        m = int(min(len(pool), max(256, budget / max(per_particle, 1e-9))))
+        stride = max(1, len(pool) // m)
+        sample = pool[::stride][:m]
+        t = time.perf_counter()
+        lost = runner.stable_aperture(ring, sample, flags=flags, turns=turns, parameters=["x", "y"], mode=mode)
+        dt = time.perf_counter() - t
+        pt = int(np.minimum(lost.astype(np.int64) + 1, turns).sum())
+        res[mode] = (pt * E / dt, stride, sample, lost, dt)
+    value, stride, sample, lost, dt = res["bench_fast"]
+    s_value, s_stride, s_sample, _, s_dt = res["bench_strict"]
+    out = {"value": value, "unit": "particle-element-turns/s", "cores": threads, "kind": "port",
            "sample": f"{len(sample)} grid points (every {stride}th of the {len(pool)}-point grid), "
-                     f"{turns} turns, {dt:.1f} s"}
+                     f"{turns} turns, {dt:.1f} s",
+           "build": "gcc " + " ".join(runner.FLAGS["bench_fast"]) + " -fopenmp",
+           "strict": {"value": s_value, "build": "gcc " + " ".join(runner.FLAGS["bench_strict"]) + " -fopenmp",
+                      "sample": f"{len(s_sample)} grid points (every {s_stride}th), {turns} turns, {s_dt:.1f} s"},
+           "host": host}
     if gpu_lost is not None:
-        same = np.asarray(gpu_lost)[::stride][:m] == lost
+        same = np.asarray(gpu_lost)[::stride][:len(sample)] == lost
         out["lost_identical"] = f"{int(same.sum())}/{len(sample)}"
     return out
 

@@ -10,6 +10,9 @@ Build modes:
               -march=native: the closest CPU analogue of the reference's
               default ``-cl-fast-relaxed-math -cl-mad-enable``, keeping NaN
               tests (a finite-math build may fold ``isnan`` away).
+  ``bench_fast``, ``bench_strict``  the CPU-baseline builds: ``fast`` and
+              ``strict`` at -O3 -march=native (timing only; the parity
+              checks use the two modes above).
 FP32 kernels add ``-fsingle-precision-constant`` (the reference's
 ``-cl-single-precision-constant``, ufo/track.py:56).
 """
@@ -31,7 +34,11 @@ _CACHE = os.environ.get("UFO_ORACLE_CACHE") or os.path.join(
 _FLAGS = {
     "strict": ["-O2", "-ffp-contract=off", "-fno-fast-math"],
     "fast": ["-O2", "-ffast-math", "-fno-finite-math-only", "-ffp-contract=fast", "-march=native"],
+    "bench_fast": ["-O3", "-march=native", "-ffast-math", "-fno-finite-math-only", "-ffp-contract=fast"],
+    "bench_strict": ["-O3", "-march=native", "-ffp-contract=off", "-fno-fast-math"],
 }
+
+FLAGS = _FLAGS
 
 
 def compile_c(src: str, mode: str = "strict", fp32: bool = False) -> ctypes.CDLL:

@@ -134,6 +134,39 @@ def test_dba_stable_aperture_subset():
     assert np.array_equal(sa.lost[regular], strict[regular])
 
 
+def test_lossy_dba_stable_aperture_subset():
+    """The lossy DA scan (sextupoles x4, k2 = +-152): 1024 points of the
+    scan's grid law (64 x 16 over x in [-20, 20] mm, y in [0, 8] mm), 1024
+    turns, launched on 256 lanes (one workgroup) so that every lane refills
+    from the work counter several times: identical loss turns on every particle whose loss
+    turn is regular (the strict, fast-math and one-ulp-moved oracle runs
+    agree and their trajectories stay within 1e-9 of each other), with loss
+    turns spread over the run (ufo/sa.py:116-122)."""
+    import torch
+    turns = 1024
+    ring = lattices.dba_ring(16, sext_scale=4.0)
+    flags = ufo.KICK | ufo.DOUBLE_PRECISION | ufo.FIVED
+    pool = lattices.da_grid(64, 16)
+    sa = ufo.StableAperture(ring, flags=flags, turns=turns, particles=len(pool), parameters=["x", "y"])
+    sa.parameters[:] = pool
+    sa.pool_device.copy_(sa._pool_host)
+    sa.launch(lanes=256)
+    gpu = sa.lost_device.cpu().numpy().view(np.uint32).astype(np.int64)
+    assert sa.lanes == 256
+    strict, regular = loss_turn_runs(
+        lambda p, m: runner.stable_aperture(ring, p, flags=flags, turns=turns, parameters=["x", "y"], mode=m),
+        pool,
+        track_fn=lambda p, m: runner.track(ring, p, flags=flags, turns=turns, parameters=["x", "y"], where=[-1],
+                                           mode=m))
+    lost = strict < turns
+    print(f"lost {lost.sum()}/{len(pool)}, loss turns 10/50/90%: {np.percentile(strict[lost], [10, 50, 90])}, "
+          f"max {strict[lost].max()}; regular {regular.sum()}; differ on regular "
+          f"{np.sum(gpu[regular] != strict[regular])}, on all {np.sum(gpu != strict)}")
+    assert 0.2 < lost.mean() < 0.7 and strict[lost].max() > 100
+    assert regular.mean() > 0.7
+    assert np.array_equal(gpu[regular], strict[regular])
+
+
 def test_dba_track_coordinates():
     """DA ring tracked 64 turns from grid points.  The grid reaches 20 mm,
     where the motion is strongly nonlinear and rounding differences grow

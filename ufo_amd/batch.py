@@ -393,7 +393,8 @@ class BatchedAperture(Simulation):
             self._stream.synchronize()
             return time.perf_counter() - t
 
-    def tune(self, candidates=((32, 8), (64, 8), (128, 8), (64, 4)), shapes=((1, 256), (2, 256), (1, 64), (2, 64)),
+    def tune(self, candidates=((32, 8), (64, 8), (128, 8), (64, 4), (64, 2)),
+             shapes=((1, 256), (2, 256), (1, 64), (2, 64)),
              sample_turns=256):
         """Pick turns per block, waves per SIMD, particles per lane and
         workgroup size by timing the first ``sample_turns`` turns: first the
@@ -401,7 +402,7 @@ class BatchedAperture(Simulation):
         lane in 256-lane workgroups, then the (particles per lane, workgroup)
         ``shapes`` at the best pair.  Cached per kernel, workload and device."""
         key = hashlib.sha1((self.kernel.src + self.ctx.device.arch).encode()).hexdigest()[:16] + \
-            f"-{self.particles}-{self.turns}-v2"
+            f"-{self.particles}-{self.turns}-v3"
         path = _tune_cache()
         try:
             cache = json.load(open(path))

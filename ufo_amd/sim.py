@@ -66,13 +66,14 @@ class Simulation:
     def src(self):
         return self.kernel.src
 
-    def resident_lanes(self, block=None) -> int:
-        """Lanes the device holds at once for this kernel (occupancy from the
-        kernel's VGPR count, 8 waves per SIMD at most, 4 SIMDs per CU)."""
+    def resident_lanes(self, block=None, waves_per_simd=8) -> int:
+        """Lanes the device holds at once for this kernel with at most
+        ``waves_per_simd`` waves per SIMD (occupancy from the kernel's VGPR
+        count, 8 waves per SIMD at most, 4 SIMDs per CU)."""
         info = self.module().info(self.kernel.name)
         v = max(8, info["vgprs"])
         v = (v + 7) // 8 * 8
-        waves = max(1, min(8, 512 // v))
+        waves = max(1, min(8, waves_per_simd, 512 // v))
         return self.ctx.device.compute_units * 4 * waves * 64
 
     def stream(self):
