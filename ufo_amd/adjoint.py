@@ -358,6 +358,29 @@ def _turn_ops(line, flags, parameters):
     return lower(line, flags, parameters)
 
 
+def merge_drifts(ops, f):
+    """Consecutive paraxial drifts as one drift of the summed length, for
+    the reverse pass only (``f`` folds an expression): a drift leaves px,
+    py, lambda_x and lambda_y unchanged, so the inverse x -= (L1 + L2) px,
+    the adjoint lambda_px += (L1 + L2) lambda_x and the length gradient
+    lambda_x px (dL1 + dL2) are those of the two drifts up to rounding (the
+    reverse reconstructs states by inverse ops, not bit for bit anyway).
+    Misalignment shifts and weak-focusing terms that fold to zero (a
+    magnet without offsets, a straight magnet) do nothing and are dropped
+    first.  The forward kernels keep every op of the reference."""
+    out = []
+    for op in ops:
+        if isinstance(op, ir.Shift) and is_zero(f(op.dx)) and is_zero(f(op.dy)):
+            continue
+        if isinstance(op, ir.Weak) and is_zero(f(op.w)):
+            continue
+        if isinstance(op, ir.Drift) and out and isinstance(out[-1], ir.Drift):
+            out[-1] = ir.Drift(ir.Expr.of(out[-1].L) + op.L)
+        else:
+            out.append(op)
+    return out
+
+
 def _reverse_layout(ops, env: Env, ntheta: int):
     """Reverse pass of one turn: the ops backwards, the repeated period as a
     descending loop; gradient registers flushed to LDS per segment."""
@@ -366,7 +389,7 @@ def _reverse_layout(ops, env: Env, ntheta: int):
     def segment(seg_ops, lv=None):
         g = _Grad()
         body: List[str] = []
-        for op in reversed(seg_ops):
+        for op in reversed(merge_drifts(seg_ops, env.f)):
             emit_reverse(op, env, body, g)
         decl = [f"ufloat {r} = {env.lit(0.0)};" for r in g.sums.values()]
         flush = []

@@ -185,7 +185,7 @@ def _reverse_turn(ops, env, diff_cols, period):
     def segment(seg_ops, in_loop):
         g = adjoint._Grad()
         body: List[str] = []
-        for op in reversed(seg_ops):
+        for op in reversed(adjoint.merge_drifts(seg_ops, env.f)):
             adjoint.emit_reverse(op, env, body, g)
         decl = [f"ufloat {r} = 0.0;" for r in g.sums.values()]
         flush = []
