@@ -45,6 +45,8 @@ def main():
     ap.add_argument("--nx", type=int, default=1024)
     ap.add_argument("--ny", type=int, default=1024)
     ap.add_argument("--turns", type=int, default=1024)
+    ap.add_argument("--sext-scale", type=float, default=1.0,
+                    help="scale of the ring's sextupole strengths for the timed scan (4: the lossy scan)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-adjoint", action="store_true")
@@ -67,7 +69,7 @@ def main():
     import ufo_amd as ufo
     from ufo_amd import lattices
 
-    ring = lattices.dba_ring(16)
+    ring = lattices.dba_ring(16, sext_scale=args.sext_scale)
     flags = ufo.KICK | ufo.DOUBLE_PRECISION | ufo.FIVED
     pool = lattices.da_grid(args.nx, args.ny)
     n = len(pool)
@@ -127,7 +129,8 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {
-            "workload": "dynamic-aperture scan: 16-period DBA ring (400 elements, teapot KICK, FIVED), "
+            "workload": "dynamic-aperture scan: 16-period DBA ring (400 elements, teapot KICK, FIVED"
+                        + (f", sextupoles x{args.sext_scale:g}" if args.sext_scale != 1.0 else "") + "), "
                         f"{args.nx}x{args.ny} grid x in [-20,20] mm, y in [0,8] mm, {args.turns} turns",
             "particles": n, "turns": args.turns, "elements": E,
             "flop_per_particle_turn": flop_pt,
@@ -563,7 +566,7 @@ def measured_traffic(args):
     """HBM bytes per launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
     summary of this workload (copy-calibrated), when the run is that
     workload; None otherwise."""
-    if (args.nx, args.ny, args.turns) != (1024, 1024, 1024):
+    if (args.nx, args.ny, args.turns, args.sext_scale) != (1024, 1024, 1024, 1.0):
         return None
     path = os.path.join(ROOT, "profiles", "ufo_sa_dba_da_scan_1M_1024turns.md")
     try:
