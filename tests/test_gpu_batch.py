@@ -134,7 +134,7 @@ def test_autotuner_picks_a_candidate():
     _, _, _, _, _, ba, _ = _setup(4, (64, 300), turns)
     ba.block_turns = None
     ba.run()
-    assert ba.block_turns in (32, 64, 128) and ba.waves_per_simd in (4, 8)
+    assert ba.block_turns in (32, 64, 128) and ba.waves_per_simd in (2, 4, 8)
     assert ba.particles_per_lane in (1, 2) and ba.workgroup in (64, 256)
     assert np.all(ba.lost <= turns)
 
