@@ -17,7 +17,9 @@ Also reported:
   cpu_baseline  the oracle (the reference's kernels restated in C, gcc
              -O3 -march=native fast-math and strict-IEEE builds, OpenMP over
              the host threads) on an evenly strided sample of the same
-             grid, as PET/s, with the host's CPU model and core counts.
+             grid, as PET/s, with the host's CPU model and core counts; and
+             beside it the reference's own OpenCL kernel on the same GPU
+             over the whole grid (``same_gpu_reference_opencl``).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
 """
@@ -647,7 +649,28 @@ def cpu_baseline(ring, flags, pool, turns, E, budget_s, gpu_lost=None):
     if gpu_lost is not None:
         same = np.asarray(gpu_lost)[::stride][:len(sample)] == lost
         out["lost_identical"] = f"{int(same.sum())}/{len(sample)}"
+        out["same_gpu_reference_opencl"] = reference_opencl_scan(ring, flags, pool, turns, E, gpu_lost)
     return out
+
+
+def reference_opencl_scan(ring, flags, pool, turns, E, gpu_lost, threads=524288):
+    """The reference's own StableAperture kernel (its templates in the
+    kernel body of ufo/sa.py, built by the device's OpenCL compiler with its
+    default options -cl-fast-relaxed-math -cl-mad-enable, ``threads``
+    work-items claiming particles with atomic_inc) on the same GPU over the
+    whole workload: kernel time (OpenCL profiling events, best of 2) and
+    whether its loss turns equal the HIP kernel's.  None without an OpenCL
+    GPU device."""
+    import numpy as np
+    from oracle import opencl
+    if not opencl.available():
+        return None
+    threads = min(threads, len(pool))
+    lost, ms = opencl.stable_aperture(ring, pool, flags, turns, ["x", "y"], threads=threads, reps=2)
+    pt = int(np.minimum(lost.astype(np.int64) + 1, turns).sum())
+    return {"kernel_ms": round(min(ms), 2), "pet_per_s": pt * E / (min(ms) * 1e-3), "threads": threads,
+            "device": opencl.device().name, "options": opencl.DEFAULT_CL_OPTIONS,
+            "lost_identical_to_hip": f"{int(np.sum(lost == np.asarray(gpu_lost)))}/{len(pool)}"}
 
 
 if __name__ == "__main__":

@@ -1,4 +1,5 @@
-"""Test helper: the reference's own OpenCL kernels on the GPU's OpenCL device.
+"""ORACLE — test infrastructure only (see refgen.py): the reference's own
+OpenCL kernels on the GPU's OpenCL device.
 
 The reference builds OpenCL-C kernels from the lattice text and runs them
 through pyopencl (ufo/sa.py:94-145, ufo/track.py:84-139).  pyopencl is not
@@ -12,8 +13,9 @@ and the reference's default options ``-cl-fast-relaxed-math -cl-mad-enable``
 (ufo/__init__.py:22; FP32 adds ``-cl-single-precision-constant``,
 ufo/track.py:53), through a small ctypes binding of libOpenCL.
 
-Test infrastructure only (the checker): the product never loads it.
-Differences from the reference's wrappers, values unchanged: the pool is a
+Only ``tests/`` and the ``cpu_baseline`` leg of ``bench.py`` (the reference's
+kernel timed on the same GPU beside the CPU baseline) use it; the product
+never loads it.  Differences from the reference's wrappers, values unchanged: the pool is a
 flat array indexed ``pool + idx * P`` (the reference declares
 ``pool[][PARAMETERS_COUNT]``) and the tracks buffer ``tracks + (idx * count +
 track_idx) * 6`` (``tracks[][count][6]``); ``threads`` must not exceed the
@@ -23,10 +25,11 @@ from __future__ import annotations
 
 import ctypes
 import os
+import time
 
 import numpy as np
 
-from oracle import refgen
+from . import refgen
 
 _CL_DEVICE_TYPE_GPU = 1 << 2
 _CL_QUEUE_PROFILING_ENABLE = 1 << 1
@@ -157,21 +160,25 @@ class Device:
     def run(self, kern, buffers, threads):
         """Launch ``run(buffers...)`` over ``threads`` work-items (local size
         left to the runtime, as pyopencl's ``None``); returns the kernel
-        time in ms from the event's profiling counters."""
+        time in ms from the event's profiling counters, or the host's wall
+        time of enqueue and wait where those read zero (under a profiler)."""
         cl = _cl()
         holders = [_vp(b) for b in buffers]
         for i, h in enumerate(holders):
             _check(cl.clSetKernelArg(kern, i, ctypes.sizeof(_vp), ctypes.byref(h)), "clSetKernelArg")
         gsize = (ctypes.c_size_t * 1)(threads)
         ev = _vp()
+        _check(cl.clFinish(self.queue), "clFinish")
+        wall = time.perf_counter()
         _check(cl.clEnqueueNDRangeKernel(self.queue, kern, 1, None, gsize, None, 0, None, ctypes.byref(ev)),
                "clEnqueueNDRangeKernel")
         _check(cl.clWaitForEvents(1, ctypes.byref(ev)), "clWaitForEvents")
+        wall = time.perf_counter() - wall
         t0, t1 = ctypes.c_uint64(), ctypes.c_uint64()
         cl.clGetEventProfilingInfo(ev, _CL_PROFILING_COMMAND_START, 8, ctypes.byref(t0), None)
         cl.clGetEventProfilingInfo(ev, _CL_PROFILING_COMMAND_END, 8, ctypes.byref(t1), None)
         cl.clReleaseEvent(ev)
-        return (t1.value - t0.value) * 1e-6
+        return (t1.value - t0.value) * 1e-6 if t1.value > t0.value else wall * 1e3
 
     def write(self, buf, array):
         _check(_cl().clEnqueueWriteBuffer(self.queue, buf, 1, 0, array.nbytes, array.ctypes.data_as(_vp), 0, None,

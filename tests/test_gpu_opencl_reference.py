@@ -3,7 +3,7 @@ MI355X: the element statements of the reference's templates wrapped in the
 kernel bodies of ufo/track.py and ufo/sa.py, built by the device's OpenCL
 compiler with the reference's default options (-cl-fast-relaxed-math
 -cl-mad-enable), launched as the reference launches them
-(tests/opencl_reference.py).  This is the comparison the reference's own
+(oracle/opencl.py).  This is the comparison the reference's own
 users would make: same inputs, the reference's kernels on the GPU they run
 on.  Kernel times of both are printed."""
 import os
@@ -23,10 +23,10 @@ pytestmark = pytest.mark.gpu
 
 @pytest.fixture(scope="module")
 def ocl():
-    import opencl_reference
-    if not opencl_reference.available():
+    from oracle import opencl
+    if not opencl.available():
         pytest.skip("no OpenCL GPU device")
-    return opencl_reference
+    return opencl
 
 
 def test_fodo16_track_full_size_against_reference_opencl(ocl):
