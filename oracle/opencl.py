@@ -25,6 +25,7 @@ from __future__ import annotations
 
 import ctypes
 import os
+import re
 import time
 
 import numpy as np
@@ -227,12 +228,18 @@ def _init(parameters, fived, dp, indent="        "):
     return text + f"{indent}oodppo = 1. / (dp + 1.);\n"
 
 
-def sa_source(line, flags, turns, parameters, particles, dp=0.0):
-    """The kernel of ufo/sa.py:94-126 over the statements of the line."""
+def sa_source(line, flags, turns, parameters, particles, dp=0.0, final_state=False):
+    """The kernel of ufo/sa.py:94-126 over the statements of the line.  With
+    ``final_state`` it also stores x, px, y, py after its last turn into a
+    fourth buffer ``state[particles][4]`` (not part of the reference: the
+    checker uses it to tell regular orbits from chaotic ones)."""
     body = "".join(refgen.line_text(line, flags=flags, parameters=parameters))
     P = max(1, len(parameters))
     src = _head(flags)
-    src += ("__kernel void run(__global ufloat *pool, __global int *pool_idx, __global uint *lost) {\n"
+    extra = ", __global ufloat *state" if final_state else ""
+    store = ("        state[(size_t)idx * 4] = x; state[(size_t)idx * 4 + 1] = px;\n"
+             "        state[(size_t)idx * 4 + 2] = y; state[(size_t)idx * 4 + 3] = py;\n") if final_state else ""
+    src += ("__kernel void run(__global ufloat *pool, __global int *pool_idx, __global uint *lost" + extra + ") {\n"
             "    __global ufloat *parameters;\n"
             "    ufloat x, y, z, px, py, dp;\n"
             "    ufloat oodppo;\n"
@@ -247,6 +254,7 @@ def sa_source(line, flags, turns, parameters, particles, dp=0.0):
             "                break;\n"
             "        }\n"
             "        lost[idx] = turn;\n"
+            + store +
             "        idx = atomic_inc(pool_idx);\n"
             f"        if (idx >= {particles})\n"
             "            break;\n"
@@ -303,10 +311,28 @@ def _options(flags, options):
     return opts
 
 
-def stable_aperture(line, pool, flags, turns, parameters, dp=0.0, threads=None, options=None, reps=1):
+_UNIT_FACTOR = re.compile(r" \* 1\.0(?![0-9])")
+
+
+def contracted(src):
+    """The kernel text with every ``* 1.0`` factor removed (the fraction of a
+    whole, unfractured element, ufo/elements.py:102-105): exact in IEEE
+    arithmetic, but ``x += px * L * 1.0`` is then ``x += px * L``, which
+    ``-cl-mad-enable`` lets the compiler fuse into one multiply-add (the
+    ROCm OpenCL compiler does not fuse it while the factor is in the text,
+    and does fuse the teapot drifts' ``x += px * (expr)``).  A build of the
+    reference's kernel that its own options permit, used as one more sample
+    of the reference's arithmetic."""
+    return _UNIT_FACTOR.sub("", src)
+
+
+def stable_aperture(line, pool, flags, turns, parameters, dp=0.0, threads=None, options=None, reps=1,
+                    final_state=False, contract=False):
     """Loss turns of the reference's StableAperture kernel on the OpenCL GPU
     (ufo/sa.py:128-145: pool_idx = threads, launch ``(threads,)``).
-    Returns (lost, kernel ms of each of ``reps`` launches)."""
+    Returns (lost, kernel ms of each of ``reps`` launches), and with
+    ``final_state`` also x, px, y, py after each particle's last turn
+    [particles, 4]."""
     dev = device()
     dt = np.float64 if flags & refgen.FLAG_DOUBLE else np.float32
     pool = np.ascontiguousarray(np.asarray(pool, dtype=dt).reshape(len(pool), -1))
@@ -314,18 +340,25 @@ def stable_aperture(line, pool, flags, turns, parameters, dp=0.0, threads=None, 
     threads = n if threads is None else int(threads)
     if not 1 <= threads <= n:
         raise ValueError(f"threads = {threads} must be in [1, {n}] (the kernel reads pool[get_global_id(0)])")
-    prog, kern = dev.build(sa_source(line, flags, turns, parameters, n, dp), _options(flags, options))
+    src = sa_source(line, flags, turns, parameters, n, dp, final_state)
+    prog, kern = dev.build(contracted(src) if contract else src, _options(flags, options))
     b_pool = dev.buffer(pool)
     b_idx = dev.buffer(nbytes=4)
     b_lost = dev.buffer(nbytes=4 * n)
+    bufs = [b_pool, b_idx, b_lost]
+    if final_state:
+        state = np.full((n, 4), np.nan, dtype=dt)
+        bufs.append(dev.buffer(state))
     ms = []
     for _ in range(reps):
         dev.write(b_idx, np.array([threads], dtype=np.int32))
-        ms.append(dev.run(kern, [b_pool, b_idx, b_lost], threads))
+        ms.append(dev.run(kern, bufs, threads))
     lost = np.empty(n, dtype=np.uint32)
     dev.read(b_lost, lost)
-    dev.release(("mem", b_pool), ("mem", b_idx), ("mem", b_lost), ("kernel", kern), ("program", prog))
-    return lost, ms
+    if final_state:
+        dev.read(bufs[3], state)
+    dev.release(*[("mem", b) for b in bufs], ("kernel", kern), ("program", prog))
+    return (lost, ms, state) if final_state else (lost, ms)
 
 
 def track(line, pool, flags, turns, parameters, where, dp=0.0, threads=None, options=None, reps=1):
@@ -365,3 +398,80 @@ def available():
         return True
     except (OSError, OpenCLError):
         return False
+
+
+def strengths_moved(line):
+    """A copy of ``line`` with every magnet strength (k1, k2, k3, k2s, k3s
+    and the entries of knl, ksl, dkn, dks) moved up by one ulp: a change of
+    the lattice at the level where implementations of the same arithmetic
+    differ (constant folding, contraction, operation order)."""
+    import copy
+    moved = copy.deepcopy(line)
+    seen = set()
+    for e in moved.flatten():
+        if id(e) in seen:          # an element object shared by several periods moves once
+            continue
+        seen.add(id(e))
+        for name in ("k1", "k2", "k3", "k2s", "k3s"):
+            v = e.__dict__.get(name)
+            if isinstance(v, float) and v != 0.0:
+                e.__dict__[name] = float(np.nextafter(v, np.inf))
+        for name in ("knl", "ksl", "dkn", "dks"):
+            v = e.__dict__.get(name)
+            if isinstance(v, list):
+                e.__dict__[name] = [float(np.nextafter(u, np.inf)) if isinstance(u, float) and u != 0.0 else u
+                                    for u in v]
+    return moved
+
+
+def reference_variants(line, pool, flags, turns, parameters, threads, dp=0.0):
+    """Loss turns and final states of the reference's StableAperture kernel
+    on the OpenCL GPU in six builds of its arithmetic: ``default`` (its
+    default options), ``strict`` (no options), ``up`` and ``down`` (default
+    options, every input moved by one ulp), ``strengths`` (the lattice with
+    every magnet strength moved by one ulp, ``strengths_moved``) and
+    ``contracted`` (the ``contracted`` text).  Returns {name: (lost, state,
+    kernel ms)}."""
+    dt = np.float64 if flags & refgen.FLAG_DOUBLE else np.float32
+    base = np.ascontiguousarray(np.asarray(pool, dtype=dt).reshape(len(pool), -1))
+    moved_line = strengths_moved(line)
+    builds = {"default": (line, base, None, False), "strict": (line, base, "", False),
+              "up": (line, np.nextafter(base, dt(np.inf)), None, False),
+              "down": (line, np.nextafter(base, dt(-np.inf)), None, False),
+              "strengths": (moved_line, base, None, False), "contracted": (line, base, None, True)}
+    out = {}
+    for name, (ln, arr, opts, fuse) in builds.items():
+        lost, ms, state = stable_aperture(ln, arr, flags, turns, parameters, dp=dp, threads=threads, options=opts,
+                                          final_state=True, contract=fuse)
+        out[name] = (lost.astype(np.int64), state.astype(np.float64), ms[0])
+    return out
+
+
+def regular_mask(variants, turns, tol=1e-9, leave_out=()):
+    """Particles whose loss turn is regular over ``variants`` (except those
+    named in ``leave_out``): every build gives the default build's loss turn
+    and, for survivors, a final state within ``tol`` of it (relative, floor
+    the rms of each coordinate over the survivors): over long runs a chaotic
+    particle near the edge of the dynamic aperture can survive in every run
+    by chance while its runs have long parted."""
+    ref_lost, ref_state, _ = variants["default"]
+    alive = ref_lost == turns
+    rms = np.sqrt(np.mean(ref_state[alive] ** 2, axis=0)) if alive.any() else np.ones(ref_state.shape[1])
+    rms = np.where(rms > 0, rms, 1.0)
+    scale = np.maximum(np.abs(np.nan_to_num(ref_state, nan=0.0, posinf=0.0, neginf=0.0)), rms)
+    regular = np.ones(len(ref_lost), dtype=bool)
+    for name, (lost, state, _) in variants.items():
+        if name == "default" or name in leave_out:
+            continue
+        regular &= lost == ref_lost
+        with np.errstate(invalid="ignore", over="ignore"):
+            d = np.abs(np.nan_to_num(state - ref_state, nan=0.0)) / scale
+        regular &= ~alive | (np.max(d, axis=1) <= tol)
+    return regular
+
+
+def loss_turn_regular(line, pool, flags, turns, parameters, threads, tol=1e-9, dp=0.0):
+    """(loss turns of the default build, regular mask over all six builds of
+    ``reference_variants``, default build ms)."""
+    v = reference_variants(line, pool, flags, turns, parameters, threads, dp)
+    return v["default"][0], regular_mask(v, turns, tol), v["default"][2]

@@ -135,3 +135,80 @@ def test_alba_thick_track_against_reference_opencl(ocl):
           f"strict CPU oracle median {np.median(e_ref):.2e} max {e_ref.max():.2e}")
     assert np.array_equal(nan_flags(hip), nan_flags(ref))
     assert np.median(e) <= 2 * np.median(e_ref) + 1e-14
+
+
+def _against_reference_variants(ocl, hip, variants, turns):
+    """Loss turns of HIP against the reference's default build on the
+    particles whose loss turn is regular over all six reference builds, and
+    the same comparison for each reference build against the default one on
+    the particles regular over the other five (leave one out).  Returns
+    (HIP mismatch rate, largest leave-one-out rate of the reference builds,
+    report)."""
+    ref = variants["default"][0]
+    regular = ocl.regular_mask(variants, turns)
+    mm = regular & (hip != ref)
+    rate = mm.sum() / max(1, regular.sum())
+    lines = [f"regular {regular.sum()} of {len(ref)}; HIP differs from the default build on {mm.sum()} "
+             f"({rate:.2e}; later {np.sum(hip[mm] > ref[mm])}, earlier {np.sum(hip[mm] < ref[mm])}), "
+             f"on {np.sum(hip != ref)} of all"]
+    loo = {}
+    for name in variants:
+        if name == "default":
+            continue
+        reg = ocl.regular_mask(variants, turns, leave_out=(name,))
+        m = reg & (variants[name][0] != ref)
+        loo[name] = m.sum() / max(1, reg.sum())
+        lines.append(f"  reference build '{name}' (left out): differs on {m.sum()} of {reg.sum()} ({loo[name]:.2e})")
+    return rate, max(loo.values()), "\n".join(lines)
+
+
+def test_lossy_scan_full_grid_against_reference_opencl(ocl):
+    """The lossy DA scan at full size (sextupoles x4, 1024 x 1024 grid, 1024
+    turns, 40 % lost) against the reference's StableAperture kernel on the
+    GPU in six builds of its arithmetic (default and strict options, inputs
+    moved one ulp up and down, strengths moved one ulp, whole-element drifts
+    fused).  At the edge of the dynamic aperture a particle can escape by
+    slow diffusion, and its loss turn then hangs on how each build rounds
+    even where all six builds agree: each reference build, left out of the
+    agreement, differs from the default build on ~2.5e-4 of the particles
+    the other five agree on (the build with moved strengths 1.2e-3).  The
+    bar: on the particles all six agree on, HIP differs from the default
+    build no more often than twice the largest of those rates, i.e. HIP's
+    arithmetic is one more build of the reference's."""
+    turns = 1024
+    ring = lattices.dba_ring(16, sext_scale=4.0)
+    flags = ufo.KICK | ufo.DOUBLE_PRECISION | ufo.FIVED
+    pool = lattices.da_grid(1024, 1024)
+    sa = ufo.StableAperture(ring, flags=flags, turns=turns, particles=len(pool), parameters=["x", "y"])
+    sa.parameters[:] = pool
+    sa.run()
+    variants = ocl.reference_variants(ring, pool, flags, turns, ["x", "y"], threads=524288)
+    rate, ref_rate, report = _against_reference_variants(ocl, sa.lost.astype(np.int64), variants, turns)
+    print(f"reference OpenCL {variants['default'][2]:.0f} ms; " + report)
+    assert rate <= 2 * ref_rate
+
+
+def test_error_seeds_against_reference_opencl(ocl):
+    """The error-seed workload, 64 of its 256 seeds (1.2 M particles, 2048
+    turns), against the reference's own way of tracking lattice variants:
+    one StableAperture kernel with every seed's errors (400 per-particle
+    parameter columns: dx, dy, b3-b6, a3-a6 of each quadrupole and
+    sextupole) bound per particle (HOWTO.md:20-22), in the six builds of the
+    lossy-scan test and with its bar."""
+    from ufo_amd.batch import BatchedAperture
+    turns = 2048
+    ring, prm, vals, counts = lattices.dba_error_model(periods=4, seeds=256)
+    seeds = 64
+    vals, counts = vals[:seeds], counts[:seeds]
+    flags = ufo.KICK | ufo.DOUBLE_PRECISION | ufo.FIVED
+    ba = BatchedAperture(ring, flags, turns, prm, vals, counts, block_turns=64)
+    grid = np.concatenate([lattices.seed_grid(int(c)) for c in counts])
+    ba.parameters[:] = grid
+    ba.run()
+    seed_of = np.repeat(np.arange(seeds), counts)
+    pool = np.concatenate([grid, vals[seed_of]], axis=1)
+    variants = ocl.reference_variants(ring, pool, flags, turns, ["x", "y"] + prm, threads=524288)
+    rate, ref_rate, report = _against_reference_variants(ocl, ba.lost.astype(np.int64), variants, turns)
+    print(f"{len(pool)} particles, {len(prm)} parameter columns, reference OpenCL {variants['default'][2]:.0f} ms; "
+          + report)
+    assert rate <= 2 * ref_rate
