@@ -640,11 +640,11 @@ def cpu_baseline(ring, flags, pool, turns, E, budget_s, gpu_lost=None):
     value, stride, sample, lost, dt = res["bench_fast"]
     s_value, s_stride, s_sample, _, s_dt = res["bench_strict"]
     out = {"value": value, "unit": "particle-element-turns/s", "cores": threads, "kind": "port",
-           "sample": f"{len(sample)} grid points (every {stride}th of the {len(pool)}-point grid), "
+           "sample": f"{len(sample)} grid points (one in {stride} of the {len(pool)}-point grid), "
                      f"{turns} turns, {dt:.1f} s",
            "build": "gcc " + " ".join(runner.FLAGS["bench_fast"]) + " -fopenmp",
            "strict": {"value": s_value, "build": "gcc " + " ".join(runner.FLAGS["bench_strict"]) + " -fopenmp",
-                      "sample": f"{len(s_sample)} grid points (every {s_stride}th), {turns} turns, {s_dt:.1f} s"},
+                      "sample": f"{len(s_sample)} grid points (one in {s_stride}), {turns} turns, {s_dt:.1f} s"},
            "host": host}
     if gpu_lost is not None:
         same = np.asarray(gpu_lost)[::stride][:len(sample)] == lost
