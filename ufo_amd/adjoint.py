@@ -248,10 +248,13 @@ def _emit_quadratic_reverse(op: ir.Kick, d, env: Env, out: List[str], g: _Grad):
     """Reverse of a pure second-order kick (a sextupole slice, w = d2 z^2):
     z^2 is formed once and serves both the inverse kick (px += Re d2 z^2,
     py -= Im d2 z^2) and the coefficient adjoint; lambda <- J^T lambda with
-    w' = 2 d2 z.  14 operations for a normal sextupole slice instead of the
-    18 of the Horner form."""
+    w' = 2 d2 z.  15 operations for a slice with both normal and skew parts
+    instead of the 18 of the Horner form; 14 for a normal sextupole slice
+    (``_emit_normal_quadratic_reverse``)."""
     d2r = None if is_zero(d[2][0]) else env.val(d[2][0])
     d2i = None if is_zero(d[2][1]) else env.val(d[2][1])
+    if d2r is not None and d2i is None and not (len(op.ks) > 2 and ir.Expr.of(op.ks[2]).params()):
+        return _emit_normal_quadratic_reverse(op, d2r, env.val(env.f(d[2][0] * 2.0)), env, out, g)
     zr, zi = env.new_tmp(), env.new_tmp()
     lines = [f"ufloat {zr} = fma(x, x, -(y * y)); ufloat {zi} = fma(x, y, y * x);"]
     px, py = "px", "py"
@@ -276,6 +279,26 @@ def _emit_quadratic_reverse(op: ir.Kick, d, env: Env, out: List[str], g: _Grad):
     ni = codegen._chain([(dr, "y", False), (di, "x", False)], None)
     lines.append(f"ufloat {a} = {nr}; ufloat {b} = {ni};")
     lines.append(f"lx = {_fma(b, 'lpy', _fma_neg(a, 'lpx', 'lx'))}; ly = {_fma(b, 'lpx', _fma(a, 'lpy', 'ly'))};")
+    out.append("{ " + " ".join(lines) + " }")
+
+
+def _emit_normal_quadratic_reverse(op: ir.Kick, d2: str, d2x2: str, env: Env, out: List[str], g: _Grad):
+    """Reverse of a normal sextupole slice, w = d2 z^2 with d2 real.
+
+    Im z^2 = 2xy: the forward's fma(x, y, y*x) rounds to exactly twice the
+    rounded product q = x*y (x*y + round(x*y) lies within a quarter ulp of
+    2 round(x*y)), so the inverse kick py -= d2 Im z^2 is fma(2 d2, -q, py),
+    bit for bit.  With u = y lpy - x lpx and v = y lpx + x lpy the lambda
+    update is lx += 2 d2 u, ly += 2 d2 v, and the coefficient adjoint
+    -lpx Re z^2 + lpy Im z^2 equals x u + y v: 14 operations."""
+    zr, q, u, v = env.new_tmp(), env.new_tmp(), env.new_tmp(), env.new_tmp()
+    lines = [f"ufloat {zr} = fma(x, x, -(y * y)); ufloat {q} = x * y;",
+             f"px = {_fma(d2, zr, 'px')}; py = {_fma_neg(d2x2, q, 'py')};",
+             f"ufloat {u} = fma(y, lpy, -(x * lpx)); ufloat {v} = fma(y, lpx, x * lpy);"]
+    if len(op.kn) > 2 and ir.Expr.of(op.kn[2]).params():
+        chi = ir.OODPPO if op.chroma else ir.E(1.0)
+        _add_grad(env, g, op.kn[2] * (chi / 2.0), [("x", u), ("y", v)], lines)
+    lines.append(f"lx = {_fma(d2x2, u, 'lx')}; ly = {_fma(d2x2, v, 'ly')};")
     out.append("{ " + " ".join(lines) + " }")
 
 
@@ -423,9 +446,6 @@ def _reverse_layout(ops, env: Env, ntheta: int):
     return out, (head, p, reps, stride)
 
 
-LDS_TABLE_MAX = 4096   # coefficient-table slots staged in LDS (32 KiB); larger tables stay in HBM
-
-
 def _table_kernel(env: Env, flat, period, reps, stride, F, S, name="ufo_adj_table"):
     """Kernel evaluating every coefficient of the table from theta, in the
     arithmetic order the tracking kernels would use inline: lane r < reps
@@ -451,15 +471,6 @@ extern "C" __global__ void __launch_bounds__(64)
 """
 
 
-def _stage(ns: int, block: int) -> str:
-    """Coefficient table into LDS (cs), or cs aliasing the table in HBM."""
-    if ns <= LDS_TABLE_MAX:
-        return (f"__shared__ ufloat cs[{ns}];\n"
-                f"    for (int i = threadIdx.x; i < {ns}; i += {block}) cs[i] = ct[i];\n"
-                f"    __syncthreads();")
-    return "const ufloat *__restrict__ cs = ct;"
-
-
 def adjoint_kernels(line, flags, turns, coords, lattice_parameters, dp=0.0, every=32, block=256):
     """(forward kernel, reverse kernel, table kernel, table layout, number of
     checkpoints) of a loss on the final state.  ``coords`` are the
@@ -468,10 +479,11 @@ def adjoint_kernels(line, flags, turns, coords, lattice_parameters, dp=0.0, ever
     the order of theta.
 
     Every coefficient that depends on theta alone is evaluated once per call
-    by the table kernel (``ufo_adj_table``) into a table ct[], which both
-    tracking kernels stage in LDS; inside the period loop the coefficients of
-    the next repetition are loaded one iteration ahead into registers, so no
-    wait on a load sits in the loop."""
+    by the table kernel (``ufo_adj_table``) into a table ct[] in global
+    memory.  Both tracking kernels index it with wave-uniform offsets only,
+    so its values arrive by scalar loads into SGPRs (no LDS instruction in
+    the turn loop); inside the period loop the coefficients of the next
+    repetition are loaded one iteration ahead."""
     if not flags & DOUBLE_PRECISION:
         raise ValueError("the adjoint tracker computes in FP64 (DOUBLE_PRECISION)")
     coords = list(coords)
@@ -493,7 +505,7 @@ def adjoint_kernels(line, flags, turns, coords, lattice_parameters, dp=0.0, ever
         env = codegen._env_for(flags, src, None)
         env.shared_offset = len(coords)
         env.table, env.period_slots = table, slots      # one table for both kernels
-        env.table_name = "cs"
+        env.table_name = "ct"
         env.prefetch = True
         env.factor_chroma = False
         env.unroll = unroll
@@ -501,7 +513,7 @@ def adjoint_kernels(line, flags, turns, coords, lattice_parameters, dp=0.0, ever
 
     P = max(1, len(coords))
     nth = max(1, len(shared))
-    pre = codegen._PRELUDE.format(ufloat="double", nan='__builtin_nan("")') + codegen.OPAQUE
+    pre = codegen._PRELUDE.format(ufloat="double", nan='__builtin_nan("")')
 
     envf = make_env()
     fbody, period = codegen._layout(ops, envf)
@@ -525,7 +537,6 @@ extern "C" __global__ void __launch_bounds__(256)
 ufo_adj_fwd(const ufloat *__restrict__ pool, const ufloat *__restrict__ ct, ufloat *__restrict__ ckpt,
             ufloat *__restrict__ xout, int *__restrict__ lost, int particles)
 {{
-    {_stage(ns, 256)}
     {codegen._decls(envf)}
     const int idx = blockIdx.x * 256 + threadIdx.x;
     if (idx >= particles) return;
@@ -558,7 +569,6 @@ ufo_adj_bwd(const ufloat *__restrict__ pool, const ufloat *__restrict__ ct, cons
 {{
     __shared__ ufloat acc[{nth} * {ACCW}];
     for (int i = threadIdx.x; i < {nth} * {ACCW}; i += {block}) acc[i] = 0.0;
-    {_stage(ns, block)}
     const int idx = blockIdx.x * {block} + threadIdx.x;
     const int slot = threadIdx.x % {ACCW};
     if (idx < particles && lost[idx] == {turns}) {{

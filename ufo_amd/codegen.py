@@ -784,16 +784,6 @@ typedef {ufloat} ufloat;
 #endif
 """
 
-# Prefetch-mode kernels only: a value the optimiser may not trace back to its
-# definition (no instruction is emitted), which keeps a prefetched,
-# loop-carried coefficient from being folded back into a load at its use.
-OPAQUE = """__device__ static inline double ufo_opaque(double v)
-{
-    __asm__("" : "+v"(v));
-    return v;
-}
-"""
-
 
 @dataclass
 class Kernel:
@@ -882,16 +872,17 @@ def prefetch_lines(env: Env, first: str, nxt: str):
     mode): K<j> holds repetition ``first``'s value before the loop; each
     iteration loads the next repetition's N<j> at its top (so the load is in
     flight while the body runs) and moves it into K<j> at its end.  Returns
-    (before the loop, loop top, loop end) lines; empty without prefetch."""
+    (before the loop, loop top, loop end) lines; empty without prefetch.
+    The repetition index is wave-uniform, so with the table in global
+    memory behind a const __restrict__ pointer the loads are scalar loads
+    into SGPRs, which the compiler is left free to schedule."""
     if not env.prefetch or not env.period_slots:
         return [], [], []
     t = env.table_name
     js = sorted(env.period_slots.values())
-    # the rotation passes through ufo_opaque: otherwise the compiler folds
-    # the loop-carried value back into a load at the top of the iteration
     pre = [f"ufloat K{j} = {t}[@PB@ + {j} + ({first}) * @PS@];" for j in js]
     top = [f"const int nrep_ = {nxt};"] + [f"const ufloat N{j} = {t}[@PB@ + {j} + nrep_ * @PS@];" for j in js]
-    end = [f"K{j} = ufo_opaque(N{j});" for j in js]
+    end = [f"K{j} = N{j};" for j in js]
     for (j, var), name in sorted(env.products.items(), key=lambda kv: kv[1]):
         pre.append(f"ufloat {name} = K{j} * {var};")
         end.append(f"{name} = K{j} * {var};")
