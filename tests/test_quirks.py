@@ -144,3 +144,52 @@ def test_indexed_strength_attributes():
         ufo.Multipole("M", knl=[0.0, 0.75]).method(flags=FL)
     with pytest.raises(IndexError):
         ufo.Multipole("M", knl=[0.0, 0.5]).method(flags=FL, k3=1.0)
+
+
+def test_constructor_signatures_bind_like_the_reference():
+    """Each element's constructor takes the reference's arguments in the
+    reference's order (ufo/elements.py:74,93,124,189,244,310,380,435,487,512):
+    positional and keyword calls bind the same way, an unknown keyword or a
+    surplus positional raises TypeError, and ``inspect.signature`` shows the
+    arguments with their defaults."""
+    import inspect
+    order = {ufo.Drift: ["label", "length"],
+             ufo.Multipole: ["label", "knl", "ksl", "dx", "dy"],
+             ufo.Quadrupole: ["label", "slices", "length", "k1", "dx", "dy", "dkn", "dks"],
+             ufo.Sbend: ["label", "slices", "length", "angle", "k1", "e1", "e2", "hgap", "fint", "dx", "dy",
+                         "dkn", "dks"],
+             ufo.Rbend: ["label", "slices", "length", "angle", "k1", "e1", "e2", "hgap", "fint", "dx", "dy",
+                         "dkn", "dks"],
+             ufo.Sextupole: ["label", "length", "slices", "k2", "k2s", "dx", "dy", "dkn", "dks"],
+             ufo.Octupole: ["label", "length", "slices", "k3", "k3s", "dx", "dy", "dkn", "dks"],
+             ufo.Wire: ["label", "x", "y", "k"],
+             ufo.Aperture: ["label", "window"],
+             ufo.Marker: ["label"]}
+    for cls, names in order.items():
+        assert list(inspect.signature(cls).parameters) == names, cls
+    q = ufo.Quadrupole("Q", 6, 0.5, 1.25, 1e-4)
+    assert (q.slices, q.length, q.k1, q.dx, q.dy, q.dkn) == (6, 0.5, 1.25, 1e-4, 0.0, [])
+    s = ufo.Sextupole("S", 0.2, None, 30.0)
+    assert (s.length, s.slices, s.k2) == (0.2, ufo.DEFAULT_SEXTUPOLE_SLICES, 30.0)
+    with pytest.raises(TypeError):
+        ufo.Drift("D", length=1.0, k1=2.0)
+    with pytest.raises(TypeError):
+        ufo.Drift("D", 1.0, 2.0)
+    with pytest.raises(TypeError):
+        ufo.Quadrupole()
+    # list arguments are copied, and the slice default is read when the
+    # element is built
+    errors = [0.0, 1e-4]
+    q = ufo.Quadrupole("Q", length=0.5, dkn=errors)
+    errors.append(5.0)
+    assert q.dkn == [0.0, 1e-4]
+    saved = ufo.DEFAULT_QUADRUPOLE_SLICES
+    try:
+        ufo.DEFAULT_QUADRUPOLE_SLICES = 3
+        assert ufo.Quadrupole("Q", length=0.5).slices == 3
+    finally:
+        ufo.DEFAULT_QUADRUPOLE_SLICES = saved
+    # repr walks the reference's parameters list; Rbend's integer defaults
+    assert repr(ufo.Drift("D", 1.5)) == "D       : DRIFT        length=1.5"
+    assert "hgap=0   fint=0" in repr(ufo.Rbend("R", length=1.0, angle=0.1))
+    assert "hgap=0.0   fint=0.0" in repr(ufo.Sbend("B", length=1.0, angle=0.1))
