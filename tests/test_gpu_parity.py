@@ -8,6 +8,7 @@ kernels) is the checker.
 import os
 
 import numpy as np
+import torch
 import pytest
 
 import ufo_amd as ufo
@@ -712,3 +713,31 @@ def test_golden_lattices(lattice, line, flags):
     # strong sextupoles and octupoles make every particle of some of these
     # lines chaotic within 30 turns: the per-particle factor 8 (parity.py)
     assert_parity(r, min_regular_frac=0.0, tail_factor=8.0)
+
+
+def test_empty_inputs_on_every_entry_point():
+    """No particles: every entry point runs without a launch and returns
+    empty results of the documented shapes (the reference cannot size an
+    OpenCL buffer of zero bytes, ufo/track.py:57); an AdjointTracker with no
+    particles gives a zero gradient."""
+    ring = lattices.fodo_ring(4)
+    flags = ufo.KICK | ufo.DOUBLE_PRECISION
+    tr = ufo.Track(ring, flags=flags, turns=5, particles=0, parameters=COORD6, where=[-1])
+    tr.run()
+    assert tr.tracks.shape == (0, 5, 6)
+    sa = ufo.StableAperture(ring, flags=flags, turns=5, particles=0, parameters=COORD6)
+    sa.run()
+    assert sa.lost.shape == (0,)
+    co = ufo.ClosedOrbit(ring, flags=flags, particles=0, parameters=["x"])
+    co.run()
+    assert co.orbits.shape == (0, 4) and co.converged.shape == (0,)
+    tm = ufo.TangentMap(ring, flags=flags, turns=3, particles=0, parameters=COORD6)
+    tm.run()
+    assert tm.state.shape == (0, 4) and tm.jacobian.shape == (0, 4, 4)
+    shared = [("QF", "k1")]
+    at = ufo.AdjointTracker(ring, flags, 10, COORD6, shared)
+    x0 = torch.zeros((0, 5), dtype=torch.float64, device=at.device)
+    xf, lost = at.forward(x0, torch.tensor([1.2], dtype=torch.float64))
+    assert xf.shape == (0, 4) and lost.shape == (0,)
+    gx0, g = at.backward(torch.zeros((0, 4), dtype=torch.float64, device=at.device))
+    assert gx0.shape == (0, 4) and g.shape == (1,) and float(g[0]) == 0.0
