@@ -42,18 +42,27 @@ FLAGS = _FLAGS
 
 
 def compile_c(src: str, mode: str = "strict", fp32: bool = False) -> ctypes.CDLL:
-    flags = ["-std=gnu11", "-fPIC", "-shared", "-fopenmp", "-w"] + _FLAGS[mode]
+    """Build the C text with the mode's flags and load it.  Compiled and
+    linked in two steps: gcc adds crtfastmath.o to a -ffast-math link, whose
+    constructor switches the loading process to flush-to-zero and
+    denormals-are-zero, which would make every later strict run and numpy
+    in the same process flush subnormals; the fast modes keep their
+    fast-math code generation without it."""
+    flags = ["-std=gnu11", "-fPIC", "-fopenmp", "-w"] + _FLAGS[mode]
     if fp32:
         flags.append("-fsingle-precision-constant")
-    key = hashlib.sha1((src + "\0" + " ".join(flags)).encode()).hexdigest()[:20]
+    key = hashlib.sha1((src + "\0" + " ".join(flags) + "\0two-step").encode()).hexdigest()[:20]
     os.makedirs(_CACHE, exist_ok=True)
     so = os.path.join(_CACHE, f"k_{key}.so")
     if not os.path.exists(so):
         c = os.path.join(_CACHE, f"k_{key}.c")
         with open(c, "w") as f:
             f.write(src)
+        obj = so + f".{os.getpid()}.o"
         tmp = so + f".{os.getpid()}.tmp"
-        subprocess.run(["gcc", *flags, "-o", tmp, c, "-lm"], check=True)
+        subprocess.run(["gcc", *flags, "-c", "-o", obj, c], check=True)
+        subprocess.run(["gcc", "-shared", "-fopenmp", "-o", tmp, obj, "-lm"], check=True)
+        os.remove(obj)
         os.replace(tmp, so)
     return ctypes.CDLL(so)
 

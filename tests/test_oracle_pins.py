@@ -139,3 +139,20 @@ def test_alba_sa_grid_howto_fp32():
     assert np.all(lost[robust] == ref[robust])
     assert np.all(lost == lost[::-1])
     assert np.sum(lost != ref) <= 6
+
+
+def test_fast_oracle_leaves_subnormals_alone():
+    """Loading a fast-math oracle build does not switch the process to
+    flush-to-zero: strict runs and numpy after it still see subnormals
+    (gcc's crtfastmath.o, added to a -ffast-math link, would set FTZ/DAZ in
+    its constructor)."""
+    import ufo_amd as ufo
+    from ufo_amd import lattices
+    ring = lattices.fodo_ring(2)
+    pool = lattices.gaussian_bunch(4)
+    for mode in ("fast", "bench_fast"):
+        runner.stable_aperture(ring, pool, flags=ufo.KICK | ufo.DOUBLE_PRECISION, turns=2,
+                               parameters=["x", "px", "y", "py", "dp"], mode=mode)
+    tiny = np.array([5e-324, 1e-310])
+    assert np.array_equal(tiny * 1.0, tiny)
+    assert np.nextafter(0.0, 1.0) * 3 == 1.5e-323
