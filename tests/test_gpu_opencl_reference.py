@@ -212,3 +212,52 @@ def test_error_seeds_against_reference_opencl(ocl):
     print(f"{len(pool)} particles, {len(prm)} parameter columns, reference OpenCL {variants['default'][2]:.0f} ms; "
           + report)
     assert rate <= 2 * ref_rate
+
+
+def test_dba_track_final_state_against_reference_opencl(ocl):
+    """The headline ring's coordinates, not only its loss turns: 8192 points
+    spanning the DA grid (128 x 64), 1024 turns, the HIP Track kernel's last
+    sample against the final state of the reference's StableAperture kernel
+    on the GPU in its six builds (``reference_variants``).  Errors relative
+    to max(|coordinate|, rms of the coordinate over the survivors).  Over
+    1024 turns of this strongly nonlinear ring one-ulp differences grow to
+    ~1e-11 typically and ~1e-9 at the edge in the reference's own builds
+    (the strict build against the default: median 9.3e-12); the bar: on the
+    particles all six builds agree on within 1e-9, HIP's median distance to
+    the default build is at most twice the largest median of the other
+    builds, and on every survivor HIP is within 4 x the largest distance of
+    any reference build from the default build (the per-particle
+    sensitivity, ufo/sa.py:110-122) + 1e-15."""
+    turns = 1024
+    ring = lattices.dba_ring(16)
+    flags = ufo.KICK | ufo.DOUBLE_PRECISION | ufo.FIVED
+    pool = lattices.da_grid(128, 64)
+    tr = ufo.Track(ring, flags=flags, turns=turns, particles=len(pool), parameters=["x", "y"], where=[-1])
+    tr.parameters[:] = pool
+    tr.run()
+    hip = np.array(tr.tracks[:, -1, :4], dtype=np.float64)
+    variants = ocl.reference_variants(ring, pool, flags, turns, ["x", "y"], threads=len(pool))
+    ref_lost, ref, _ = variants["default"]
+    alive = ref_lost == turns
+    assert np.array_equal(np.isfinite(hip[:, 0]), alive)
+    rms = np.sqrt(np.mean(ref[alive] ** 2, axis=0))
+    scale = np.maximum(np.abs(ref), rms)
+
+    def dist(state):
+        return np.max(np.abs(state - ref) / scale, axis=1)
+
+    regular = ocl.regular_mask(variants, turns)
+    e_hip = dist(hip)
+    spread = {name: dist(v[1]) for name, v in variants.items() if name != "default"}
+    sens = np.max(np.stack(list(spread.values())), axis=0)
+    worst_median = max(np.median(e[regular]) for e in spread.values())
+    ratio = e_hip[alive] / np.maximum(sens[alive], 1e-300)
+    print(f"regular {regular.sum()} of {alive.sum()} survivors; HIP vs default build: median "
+          f"{np.median(e_hip[regular]):.2e}, p90 {np.percentile(e_hip[regular], 90):.2e}, max "
+          f"{e_hip[regular].max():.2e}; within 1e-12: HIP {np.sum(e_hip[regular] <= 1e-12)}, "
+          + ", ".join(f"{n} {np.sum(e[regular] <= 1e-12)} (median {np.median(e[regular]):.2e})"
+                      for n, e in spread.items())
+          + f"; HIP / sensitivity: median {np.median(ratio):.3f}, max {ratio.max():.2f}")
+    assert regular.mean() > 0.9
+    assert np.median(e_hip[regular]) <= 2 * worst_median
+    assert np.all(e_hip[alive] <= 4 * sens[alive] + 1e-15)
