@@ -137,3 +137,24 @@ def test_z_function_matches_brute_force():
             while i + k < len(s) and s[k] == s[i + k]:
                 k += 1
             assert z[i] == k, (s, i)
+
+
+def test_gradient_accumulators_cleared_behind_a_barrier():
+    """The reverse kernels clear their LDS gradient accumulators and then
+    wait at a workgroup barrier before any wave adds to them (without it a
+    wave that starts late can clear a slot another wave has already added
+    to), and wait again before the per-workgroup sums are read."""
+    from ufo_amd import adjoint, batch_adjoint
+    ring = lattices.fodo_ring(4, distinct=True)
+    flags = ufo.KICK | ufo.DOUBLE_PRECISION
+    coords = ["x", "px", "y", "py", "dp"]
+    shared = [("QF0", "k1"), ("SD1", "k2")]
+    kb = adjoint.adjoint_kernels(ring, flags, 20, coords, shared, every=8)[1]
+    kbb = batch_adjoint.batched_adjoint_kernels(ring, flags, 20, coords, shared, shared, every=8)[1]
+    for src in (kb.src, kbb.src):
+        clear = src.index("acc[i] = 0.0")
+        first_add = src.index("unsafeAtomicAdd(&acc[")
+        last_add = src.rindex("unsafeAtomicAdd(&acc[")
+        read = src.index("s += acc[")
+        assert "__syncthreads()" in src[clear:first_add]
+        assert "__syncthreads()" in src[last_add:read]

@@ -569,6 +569,7 @@ ufo_adj_bwd(const ufloat *__restrict__ pool, const ufloat *__restrict__ ct, cons
 {{
     __shared__ ufloat acc[{nth} * {ACCW}];
     for (int i = threadIdx.x; i < {nth} * {ACCW}; i += {block}) acc[i] = 0.0;
+    __syncthreads();   // every slot cleared before any wave adds to it
     const int idx = blockIdx.x * {block} + threadIdx.x;
     const int slot = threadIdx.x % {ACCW};
     if (idx < particles && lost[idx] == {turns}) {{
