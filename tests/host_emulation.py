@@ -241,3 +241,109 @@ def run_seed_orbits(kernel, table, guess):
     lib.host_run(_ptr(table), _ptr(guess), _ptr(orbits), _ptr(steps), _ptr(conv), ctypes.c_int(n),
                  ctypes.c_int(kernel.block))
     return orbits, steps, conv != 0
+
+
+# ---------------------------------------------------------------------------
+# Kernels with workgroup barriers and LDS atomics (the adjoint kernels): one
+# host thread per lane, a C++20 barrier per workgroup, std::atomic_ref for
+# the FP64 atomic adds.
+# ---------------------------------------------------------------------------
+
+_THREADED_PRELUDE = r"""
+#include <cmath>
+#include <cstddef>
+#include <atomic>
+#include <barrier>
+#include <thread>
+#include <vector>
+using std::fma; using std::sqrt; using std::fabs; using std::sin; using std::cos; using std::tan;
+using std::sinh; using std::cosh; using std::isnan; using std::rint; using std::fmod; using std::ldexp;
+typedef double ufloat;   // the adjoint kernels compute in FP64 only
+struct ufo_dim3 { unsigned x, y, z; };
+static thread_local ufo_dim3 blockIdx, threadIdx;
+static std::barrier<> *ufo_workgroup;
+#define __global__
+#define __device__
+#define __shared__ static
+#define __launch_bounds__(n)
+static inline void __syncthreads() { ufo_workgroup->arrive_and_wait(); }
+static inline void unsafeAtomicAdd(double *p, double v) { std::atomic_ref<double>(*p).fetch_add(v); }
+template <class F> static void ufo_grid(int grid, int block, F body)
+{
+    for (int b = 0; b < grid; ++b) {
+        std::barrier<> wg(block);
+        ufo_workgroup = &wg;
+        std::vector<std::thread> lanes;
+        for (int t = 0; t < block; ++t)
+            lanes.emplace_back([=] { blockIdx = {unsigned(b), 0, 0}; threadIdx = {unsigned(t), 0, 0}; body(); });
+        for (auto &l : lanes) l.join();
+    }
+}
+"""
+
+_ADJ_DRIVER = r"""
+extern "C" void host_table(int grid, const ufloat *th, ufloat *ct)
+{ ufo_grid(grid, 64, [=] { %(table)s(th, ct); }); }
+extern "C" void host_fwd(int grid, const ufloat *pool, const ufloat *ct, ufloat *ckpt, ufloat *xout, int *lost, int n)
+{ ufo_grid(grid, 256, [=] { %(fwd)s(pool, ct, ckpt, xout, lost, n); }); }
+extern "C" void host_bwd(int grid, int block, const ufloat *pool, const ufloat *ct, const ufloat *ckpt,
+                         const ufloat *lam, const int *lost, ufloat *gx0, ufloat *gpart, int n)
+{ ufo_grid(grid, block, [=] { %(bwd)s(pool, ct, ckpt, lam, lost, gx0, gpart, n); }); }
+"""
+
+
+class HostAdjoint:
+    """The kernels of ``adjoint.adjoint_kernels`` built for the host and
+    launched as ``AdjointTracker`` launches them: ``forward(x0, theta)`` ->
+    (final state [n, 4], lost [n], saved), ``backward(lam, saved)`` ->
+    (dL/dx0 [n, 4], dL/dtheta)."""
+
+    def __init__(self, line, flags, turns, coords, lattice_parameters, every=8):
+        from ufo_amd import adjoint
+        self.kfwd, self.kbwd, self.ktab, self.layout, self.ncheck = adjoint.adjoint_kernels(
+            line, flags, turns, coords, lattice_parameters, every=every)
+        self.turns, self.npar = turns, max(1, len(lattice_parameters))
+        # one translation unit, each kernel's source in a namespace of its
+        # own (the sources repeat their helper functions)
+        kernels = (self.ktab, self.kfwd, self.kbwd)
+        body = "\n".join(f"namespace k{i} {{\n{k.src}\n}}" for i, k in enumerate(kernels))
+        src = _THREADED_PRELUDE + body + _ADJ_DRIVER % dict(table=f"k0::{self.ktab.name}", fwd=f"k1::{self.kfwd.name}",
+                                                            bwd=f"k2::{self.kbwd.name}")
+        key = hashlib.sha1(src.encode()).hexdigest()[:20]
+        os.makedirs(_CACHE, exist_ok=True)
+        so = os.path.join(_CACHE, f"a_{key}.so")
+        if not os.path.exists(so):
+            cpp = os.path.join(_CACHE, f"a_{key}.cpp")
+            with open(cpp, "w") as f:
+                f.write(src)
+            tmp = so + f".{os.getpid()}"
+            subprocess.run(["g++", "-std=c++20", "-O2", "-ffp-contract=off", "-fno-fast-math", "-fPIC", "-shared",
+                            "-pthread", "-w", "-o", tmp, cpp], check=True)
+            os.replace(tmp, so)
+        self.lib = ctypes.CDLL(so)
+
+    def forward(self, x0, theta):
+        x0 = np.ascontiguousarray(x0, dtype=np.float64)
+        n = x0.shape[0]
+        th = np.ascontiguousarray(np.asarray(theta, dtype=np.float64).reshape(-1))
+        ct = np.zeros(self.layout["nslots"])
+        grid = max(1, (max(self.layout["reps"], 1) + 63) // 64)
+        self.lib.host_table(ctypes.c_int(grid), _ptr(th), _ptr(ct))
+        ckpt = np.full((self.ncheck, 4, n), np.nan)
+        xf = np.full((n, 4), np.nan)
+        lost = np.full(n, -1, dtype=np.int32)
+        self.lib.host_fwd(ctypes.c_int((n + 255) // 256), _ptr(x0), _ptr(ct), _ptr(ckpt), _ptr(xf), _ptr(lost),
+                          ctypes.c_int(n))
+        return xf, lost, (x0, ct, ckpt, lost)
+
+    def backward(self, lam, saved):
+        x0, ct, ckpt, lost = saved
+        n = x0.shape[0]
+        lam = np.ascontiguousarray(np.nan_to_num(lam, nan=0.0), dtype=np.float64)
+        block = self.kbwd.block
+        nb = max(1, (n + block - 1) // block)
+        gx0 = np.zeros((n, 4))
+        gpart = np.full((nb, self.npar), np.nan)
+        self.lib.host_bwd(ctypes.c_int(nb), ctypes.c_int(block), _ptr(x0), _ptr(ct), _ptr(ckpt), _ptr(lam),
+                          _ptr(lost), _ptr(gx0), _ptr(gpart), ctypes.c_int(n))
+        return gx0, gpart.sum(axis=0)

@@ -1,0 +1,90 @@
+"""The adjoint kernels (``adjoint.adjoint_kernels``) built for the host
+(one thread per lane, workgroup barriers and FP64 atomics emulated,
+``host_emulation.HostAdjoint``): the reverse pass's dL/dtheta and dL/dx0
+against central differences of the forward pass, for every element kind the
+adjoint differentiates, on the CPU.  The GPU tests check the same kernels on
+the device at full size."""
+import numpy as np
+import pytest
+
+import ufo_amd as ufo
+from ufo_amd import lattices
+
+from host_emulation import HostAdjoint
+
+COORDS = ["x", "px", "y", "py", "dp"]
+
+
+def _check(line, flags, shared, theta, bunch, turns=24, tol=1e-6):
+    adj = HostAdjoint(line, flags, turns, COORDS, shared, every=8)
+    rng = np.random.default_rng(7)
+    w = rng.normal(size=(len(bunch), 4))
+    xf, lost, saved = adj.forward(bunch, theta)
+    assert np.all(lost == turns)
+
+    def loss(t, x0=bunch):
+        return float(np.sum(w * adj.forward(x0, t)[0]))
+
+    gx0, g = adj.backward(w, saved)
+    # central differences with a step relative to each parameter's size
+    h = 1e-6 * np.maximum(np.abs(theta), 1e-2)
+    eye = np.eye(len(theta))
+    fd = np.array([(loss(theta + h[i] * eye[i]) - loss(theta - h[i] * eye[i])) / (2 * h[i])
+                   for i in range(len(theta))])
+    err = np.abs(g - fd) / np.max(np.abs(fd))
+    # dL/dx0 of three particles, every coordinate
+    gx_fd = np.zeros((3, 4))
+    for p in range(3):
+        for c in range(4):
+            hx = 1e-8
+            up, dn = bunch.copy(), bunch.copy()
+            up[p, c] += hx
+            dn[p, c] -= hx
+            gx_fd[p, c] = (loss(theta, up) - loss(theta, dn)) / (2 * hx)
+    err_x = np.abs(gx0[:3] - gx_fd) / np.max(np.abs(gx_fd))
+    print(f"dL/dtheta vs central differences: max rel {err.max():.2e}; dL/dx0: {err_x.max():.2e}")
+    assert err.max() < tol
+    assert err_x.max() < tol
+    return g
+
+
+def test_fodo_kick_strengths_and_misalignment():
+    """Teapot quadrupoles (k1), thin sextupole multipoles (knl[2] as k2, the
+    14-operation normal-sextupole reverse) and a quadrupole misalignment."""
+    ring = lattices.fodo_ring(4, distinct=True)
+    shared = [("QF0", "k1"), ("QD1", "k1"), ("SF2", "k2"), ("SD3", "k2"), ("QF1", "dx"), ("QD2", "dy")]
+    theta = np.array([1.2, -1.2, 0.8, -1.4, 2e-4, -1e-4])
+    bunch = lattices.gaussian_bunch(40, seed=1, sx=3e-3, sy=3e-3)
+    _check(ring, ufo.KICK | ufo.DOUBLE_PRECISION, shared, theta, bunch)
+
+
+def test_thick_elements():
+    """Thick quadrupoles and combined-function bends (matrices, dispersion,
+    pole faces and fringe): quadrupole and bend k1 and a bend angle."""
+    out = []
+    for i in range(2):
+        out += [ufo.Quadrupole(f"QF{i}", length=0.4, k1=1.2), ufo.Drift(f"DA{i}", length=1.5),
+                ufo.Sbend(f"B{i}", length=1.0, angle=0.1, k1=-0.2, e1=0.05, e2=0.05, hgap=0.02, fint=0.5),
+                ufo.Drift(f"DB{i}", length=1.0), ufo.Quadrupole(f"QD{i}", length=0.4, k1=-1.4),
+                ufo.Drift(f"DC{i}", length=1.5)]
+    ring = ufo.Line("R", out)
+    shared = [("QF0", "k1"), ("QD1", "k1"), ("B0", "k1"), ("B1", "angle")]
+    theta = np.array([1.2, -1.4, -0.2, 0.1])
+    bunch = lattices.gaussian_bunch(40, seed=2, sx=2e-3, sy=2e-3)
+    _check(ring, ufo.DOUBLE_PRECISION, shared, theta, bunch)
+
+
+def test_skew_and_higher_order_kicks():
+    """A multipole with normal and skew entries up to order 3 (the Horner
+    reverse), a teapot sextupole with a skew error (the two-part quadratic
+    reverse) and an octupole, each strength a parameter."""
+    cell = [ufo.Quadrupole("QF", length=0.5, k1=1.2), ufo.Drift("D1", length=1.0),
+            ufo.Multipole("M", knl=[0.0, 0.0, 0.6, 5.0], ksl=[0.0, 0.05, 0.3]),
+            ufo.Sextupole("S", length=0.2, k2=4.0, dks=[0.0, 0.0, 0.5]), ufo.Drift("D2", length=1.0),
+            ufo.Quadrupole("QD", length=0.5, k1=-1.2), ufo.Octupole("O", length=0.2, k3=30.0),
+            ufo.Drift("D3", length=2.0)]
+    ring = ufo.Line("R", cell * 3)
+    shared = [("M", "k2"), ("M", "k3"), ("M", "k1s"), ("M", "k2s"), ("S", "k2"), ("S", "dk2s"), ("O", "k3")]
+    theta = np.array([0.6, 5.0, 0.05, 0.3, 4.0, 0.5, 30.0])
+    bunch = lattices.gaussian_bunch(40, seed=3, sx=2e-3, sy=2e-3)
+    _check(ring, ufo.KICK | ufo.DOUBLE_PRECISION, shared, theta, bunch)
