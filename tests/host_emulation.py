@@ -347,3 +347,66 @@ class HostAdjoint:
         self.lib.host_bwd(ctypes.c_int(nb), ctypes.c_int(block), _ptr(x0), _ptr(ct), _ptr(ckpt), _ptr(lam),
                           _ptr(lost), _ptr(gx0), _ptr(gpart), ctypes.c_int(n))
         return gx0, gpart.sum(axis=0)
+
+
+_BADJ_DRIVER = r"""
+extern "C" void host_fwd(int grid, const ufloat *pool, const ufloat *table, const int *ts, const int *t0,
+                         const int *tc, ufloat *ckpt, ufloat *xout, int *lost, int n)
+{ ufo_grid(grid, 64, [=] { %(fwd)s(pool, table, ts, t0, tc, ckpt, xout, lost, n); }); }
+extern "C" void host_bwd(int grid, const ufloat *pool, const ufloat *table, const int *ts, const int *t0,
+                         const int *tc, const ufloat *ckpt, const ufloat *lam, const int *lost, ufloat *gx0,
+                         ufloat *gseed, int n)
+{ ufo_grid(grid, 64, [=] { %(bwd)s(pool, table, ts, t0, tc, ckpt, lam, lost, gx0, gseed, n); }); }
+"""
+
+
+class HostBatchedAdjoint:
+    """The two kernels of a ``batch_adjoint.BatchedAdjoint`` (built on the
+    CPU; it needs no device until it launches) on the host, launched over
+    its 64-particle tiles as it launches them: ``forward(x0)`` -> (final
+    state, lost, saved), ``backward(lam, saved)`` -> (dL/dx0, dL/dtheta per
+    seed [seeds, D])."""
+
+    def __init__(self, batched):
+        self.b = batched
+        kernels = (batched.kfwd, batched.kbwd)
+        body = "\n".join(f"namespace k{i} {{\n{k.src}\n}}" for i, k in enumerate(kernels))
+        src = _THREADED_PRELUDE.replace("#define __shared__ static", "#define __shared__ static\n#define __constant__")
+        src += body + _BADJ_DRIVER % dict(fwd=f"k0::{batched.kfwd.name}", bwd=f"k1::{batched.kbwd.name}")
+        key = hashlib.sha1(src.encode()).hexdigest()[:20]
+        os.makedirs(_CACHE, exist_ok=True)
+        so = os.path.join(_CACHE, f"b_{key}.so")
+        if not os.path.exists(so):
+            cpp = os.path.join(_CACHE, f"b_{key}.cpp")
+            with open(cpp, "w") as f:
+                f.write(src)
+            tmp = so + f".{os.getpid()}"
+            subprocess.run(["g++", "-std=c++20", "-O2", "-ffp-contract=off", "-fno-fast-math", "-fPIC", "-shared",
+                            "-pthread", "-w", "-o", tmp, cpp], check=True)
+            os.replace(tmp, so)
+        self.lib = ctypes.CDLL(so)
+        self.table = np.ascontiguousarray(batched.table, dtype=np.float64)
+        self.tiles = [np.ascontiguousarray(t, dtype=np.int32) for t in batched._tiles]
+
+    def forward(self, x0):
+        x0 = np.ascontiguousarray(x0, dtype=np.float64)
+        n = self.b.particles
+        assert x0.shape == (n, len(self.b.coords))
+        ckpt = np.full((self.b.ncheck, 4, max(1, n)), np.nan)
+        xf = np.full((n, 4), np.nan)
+        lost = np.full(n, -1, dtype=np.int32)
+        ts, t0, tc = self.tiles
+        self.lib.host_fwd(ctypes.c_int(len(ts)), _ptr(x0), _ptr(self.table), _ptr(ts), _ptr(t0), _ptr(tc),
+                          _ptr(ckpt), _ptr(xf), _ptr(lost), ctypes.c_int(n))
+        return xf, lost, (x0, ckpt, lost)
+
+    def backward(self, lam, saved):
+        x0, ckpt, lost = saved
+        n = self.b.particles
+        lam = np.ascontiguousarray(np.nan_to_num(lam, nan=0.0), dtype=np.float64)
+        gx0 = np.zeros((n, 4))
+        gseed = np.zeros((len(self.b.counts), self.b.ndiff))
+        ts, t0, tc = self.tiles
+        self.lib.host_bwd(ctypes.c_int(len(ts)), _ptr(x0), _ptr(self.table), _ptr(ts), _ptr(t0), _ptr(tc),
+                          _ptr(ckpt), _ptr(lam), _ptr(lost), _ptr(gx0), _ptr(gseed), ctypes.c_int(n))
+        return gx0, gseed
