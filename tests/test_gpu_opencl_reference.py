@@ -261,3 +261,38 @@ def test_dba_track_final_state_against_reference_opencl(ocl):
     assert regular.mean() > 0.9
     assert np.median(e_hip[regular]) <= 2 * worst_median
     assert np.all(e_hip[alive] <= 4 * sens[alive] + 1e-15)
+
+
+@pytest.mark.parametrize("case", ["fodo16_kick", "fodo16_thick", "dba_kick_fived"])
+def test_fp32_track_against_reference_opencl(ocl, case):
+    """The reference's default precision (no DOUBLE_PRECISION flag: FP32
+    state and constants, ``-cl-single-precision-constant``): Track through
+    the reference's OpenCL kernel on the GPU.  In FP32 the rounding of every
+    operation moves trajectories by ~1e-6 per turn, so over these runs two
+    correct builds part by ~1e-4-1e-3: the bar is identical lost/alive flags
+    and a median distance between HIP and the reference kernel at most twice
+    the larger median distance of two of the reference's own builds from the
+    strict CPU oracle (its OpenCL kernel, and the oracle built with fast-math
+    contraction)."""
+    ring, flags, turns, prm, pool = {
+        "fodo16_kick": (lattices.fodo_ring(16), ufo.KICK, 1000, ["x", "px", "y", "py", "dp"],
+                        lattices.gaussian_bunch(4096, seed=0)),
+        "fodo16_thick": (lattices.fodo_ring(16), 0, 1000, ["x", "px", "y", "py", "dp"],
+                         lattices.gaussian_bunch(4096, seed=0)),
+        "dba_kick_fived": (lattices.dba_ring(16), ufo.KICK | ufo.FIVED, 200, ["x", "y"], lattices.da_grid(64, 64)),
+    }[case]
+    tr = ufo.Track(ring, flags=flags, turns=turns, particles=len(pool), parameters=prm, where=[-1])
+    tr.parameters[:] = pool
+    tr.run()
+    hip = np.array(tr.tracks, dtype=np.float64)
+    ref, _ = ocl.track(ring, pool, flags, turns, prm, [-1], threads=len(pool))
+    ref = ref.astype(np.float64)
+    strict, fast = (runner.track(ring, pool, flags=flags, turns=turns, parameters=prm, where=[-1],
+                                 mode=m).astype(np.float64) for m in ("strict", "fast"))
+    e = np.nanmedian(particle_err(hip, ref))
+    spread = {"reference OpenCL": np.nanmedian(particle_err(ref, strict)),
+              "fast-math oracle": np.nanmedian(particle_err(fast, strict))}
+    print(f"{case}: HIP vs reference OpenCL median {e:.2e}; from the strict oracle: "
+          + ", ".join(f"{k} {v:.2e}" for k, v in spread.items()))
+    assert np.array_equal(nan_flags(hip), nan_flags(ref))
+    assert e <= 2 * max(spread.values())
