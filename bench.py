@@ -168,14 +168,15 @@ def main():
     if not args.no_co_optics:
         out["co_optics"] = co_optics_timing(cpu=not args.no_cpu)
         _progress("ClosedOrbit and Optics done")
+    side_cpu = 0.0 if args.no_cpu else args.cpu_seconds / 4
     if not args.no_adjoint:
-        out["adjoint"] = adjoint_timing()
+        out["adjoint"] = adjoint_timing(cpu_seconds=side_cpu)
         _progress("adjoint done")
     if not args.no_batched:
-        out["batched"] = batched_timing()
+        out["batched"] = batched_timing(cpu_seconds=side_cpu)
         _progress("batched seeds done")
     if not args.no_seed_adjoint:
-        out["seed_adjoint"] = seed_adjoint_timing()
+        out["seed_adjoint"] = seed_adjoint_timing(cpu_seconds=side_cpu)
         _progress("batched adjoint done")
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(ring, flags, pool, args.turns, E, args.cpu_seconds, gpu_lost=lost)
@@ -345,12 +346,15 @@ def co_optics_timing(reps=5, cpu=True):
     return out
 
 
-def batched_timing(seeds=256, turns=2048, reps=2):
+def batched_timing(seeds=256, turns=2048, reps=2, cpu_seconds=0.0):
     """Error-seed scan in one launch sequence: the 4-period DBA ring with
     256 seeds of misalignments (50 um) and b3-b6/a3-a6 errors (1e-4 at 10 mm),
     particle counts per seed uniform in [4096, 32768] on the DA grid law,
     2048 turns in hipGraph-captured blocks (block length and occupancy from
-    the autotuner).  Timed from the graph launch to its completion."""
+    the autotuner).  Timed from the graph launch to its completion.  With
+    ``cpu_seconds``: the oracle on a sample of seeds 0-1, every seed's errors
+    bound per particle as the reference tracks lattice variants, and the
+    whole workload's host time extrapolated from its rate."""
     import numpy as np
     import torch
     import ufo_amd as ufo
@@ -372,23 +376,47 @@ def batched_timing(seeds=256, turns=2048, reps=2):
     lost = ba._buffers()["lost"][:ba.particles].cpu().numpy()
     pt = int(np.minimum(lost.astype(np.int64) + 1, turns).sum())
     dt = float(np.mean(times))
-    return {"workload": f"{seeds} error seeds of the 4-period DBA ring ({ba.kernel.elements} elements), "
-                        f"{ba.particles} particles (per-seed counts in [4096, 32768]), {turns} turns",
-            "seconds": dt, "pet_per_s": pt * ba.kernel.elements / dt,
-            "tflops": pt * ba.kernel.flops_per_turn / dt / 1e12,
-            "flop_per_particle_turn": ba.kernel.flops_per_turn, "executed_particle_turns": pt,
-            "survivors": int((lost == turns).sum()), "block_turns": ba.block_turns,
-            "waves_per_simd": ba.waves_per_simd, "particles_per_lane": ba.particles_per_lane,
-            "workgroup": ba.workgroup}
+    out = {"workload": f"{seeds} error seeds of the 4-period DBA ring ({ba.kernel.elements} elements), "
+                       f"{ba.particles} particles (per-seed counts in [4096, 32768]), {turns} turns",
+           "seconds": dt, "pet_per_s": pt * ba.kernel.elements / dt,
+           "tflops": pt * ba.kernel.flops_per_turn / dt / 1e12,
+           "flop_per_particle_turn": ba.kernel.flops_per_turn, "executed_particle_turns": pt,
+           "survivors": int((lost == turns).sum()), "block_turns": ba.block_turns,
+           "waves_per_simd": ba.waves_per_simd, "particles_per_lane": ba.particles_per_lane,
+           "workgroup": ba.workgroup}
+    if cpu_seconds:
+        # the reference's own way of tracking lattice variants: one kernel with
+        # every seed's errors bound per particle (HOWTO.md:20-22), seeds 0-1
+        from oracle import runner
+        first = int(counts[0] + counts[1])
+        pool = np.concatenate([lattices.seed_grid(int(c)) for c in counts[:2]])
+        pool = np.concatenate([pool, np.repeat(vals[:2], counts[:2], axis=0)], axis=1)
+        flags = ufo.KICK | ufo.DOUBLE_PRECISION | ufo.FIVED
+
+        def run_rows(idx):
+            return _executed(runner.stable_aperture(ring, pool[idx], flags=flags, turns=turns,
+                                                    parameters=["x", "y"] + prm, mode="bench_fast"), turns)
+
+        rate, m, secs = _cpu_sample_rate(run_rows, first, ba.kernel.elements, cpu_seconds)
+        out["cpu_baseline"] = {"pet_per_s": rate, "kind": "port", "build": "bench_fast",
+                               "sample": f"{m} of the {first} particles of seeds 0-1, {len(prm)} per-particle "
+                                         f"parameter columns, {turns} turns, {secs:.1f} s",
+                               "seconds_extrapolated": pt * ba.kernel.elements / rate,
+                               "gpu_over_cpu": pt * ba.kernel.elements / dt / rate}
+    return out
 
 
-def seed_adjoint_timing(seeds=64, per_seed=8192, turns=1024, every=32):
+def seed_adjoint_timing(seeds=64, per_seed=8192, turns=1024, every=32, cpu_seconds=0.0):
     """Batched sextupole-optimisation step: the 4-period error model, 64
     seeds x 8192 particles (Gaussian sigma_x = 4 mm, sigma_y = 2 mm,
     sigma_px,py = 0.1 mrad, sigma_delta = 2e-3) launched around each seed's
     closed orbit (Newton on the tangent kernel, tolerance 1e-13 m),
     1024 turns, per-seed gradient of the per-seed amplitude/survival loss with
-    respect to the 8 sextupole family strengths."""
+    respect to the 8 sextupole family strengths.  With ``cpu_seconds``: the
+    oracle's rate on a sample of seed 0 (its errors bound per particle), and
+    the host time of a forward pass over every seed and of the
+    central-difference gradients (two passes per family) extrapolated from
+    it."""
     import numpy as np
     import torch
     import ufo_amd as ufo
@@ -439,15 +467,38 @@ def seed_adjoint_timing(seeds=64, per_seed=8192, turns=1024, every=32):
             bw.append(ev[2].elapsed_time(ev[3]))
     f_ms, b_ms = float(np.median(fw)), float(np.median(bw))
     check = seed_gradient_check(ba, ring, flags, turns, prm, vals, fam, x0, seg, seeds, per_seed)
-    return {"workload": f"{seeds} seeds x {per_seed} particles, 4-period DBA with errors, 6D, {turns} turns, "
-                        f"gradient w.r.t. {len(fam)} sextupole families per seed",
-            "closed_orbit_ms": round(co_ms, 2), "newton_steps_max": int(co.steps.max()),
-            "seeds_converged": int(co.converged.sum()),
-            "forward_ms": round(f_ms, 2), "reverse_ms": round(b_ms, 2),
-            "gradient_over_forward": round((f_ms + b_ms) / f_ms, 3),
-            "survivors": int((lost == turns).sum().item()), "particles": n,
-            "hiprtc_compile_s": round(compile_s, 1), "gradient_finite": bool(torch.isfinite(gseed).all().item()),
-            **check}
+    out = {"workload": f"{seeds} seeds x {per_seed} particles, 4-period DBA with errors, 6D, {turns} turns, "
+                       f"gradient w.r.t. {len(fam)} sextupole families per seed",
+           "closed_orbit_ms": round(co_ms, 2), "newton_steps_max": int(co.steps.max()),
+           "seeds_converged": int(co.converged.sum()),
+           "forward_ms": round(f_ms, 2), "reverse_ms": round(b_ms, 2),
+           "gradient_over_forward": round((f_ms + b_ms) / f_ms, 3),
+           "survivors": int((lost == turns).sum().item()), "particles": n,
+           "hiprtc_compile_s": round(compile_s, 1), "gradient_finite": bool(torch.isfinite(gseed).all().item()),
+           **check}
+    if cpu_seconds:
+        # seed 0 through the oracle with its errors bound per particle; the
+        # host's route to the per-seed gradients is central differences, two
+        # forward passes of every seed per family
+        from oracle import runner
+        E = ba.kernel.elements
+        pool0 = np.concatenate([x0[:per_seed].cpu().numpy(), np.repeat(vals[:1], per_seed, axis=0)], axis=1)
+
+        def run_rows(idx):
+            return _executed(runner.stable_aperture(ring, pool0[idx], flags=flags, turns=turns,
+                                                    parameters=["x", "px", "y", "py", "dp"] + prm,
+                                                    mode="bench_fast"), turns)
+
+        rate, m, secs = _cpu_sample_rate(run_rows, per_seed, E, cpu_seconds)
+        pt = int(torch.clamp(lost.to(torch.int64) + 1, max=turns).sum())
+        fwd_s = pt * E / rate
+        out["cpu_baseline"] = {"pet_per_s": rate, "kind": "port", "build": "bench_fast",
+                               "sample": f"{m} of seed 0's {per_seed} particles, {len(prm)} per-particle parameter "
+                                         f"columns, {turns} turns, {secs:.1f} s",
+                               "forward_seconds_extrapolated": fwd_s,
+                               "central_difference_gradient_seconds_extrapolated": 2 * len(fam) * fwd_s,
+                               "gpu_gradient_speedup": 2 * len(fam) * fwd_s / ((f_ms + b_ms) * 1e-3)}
+    return out
 
 
 def seed_gradient_check(ba, ring, flags, turns, prm, vals, fam, x0, seg, seeds, per_seed, amp_max=30.0, h=1e-3):
@@ -506,7 +557,7 @@ def seed_gradient_check(ba, ring, flags, turns, prm, vals, fam, x0, seg, seeds, 
                                  f"{int(alive.sum())} survivors, amplification <= {amp_max:g})"}
 
 
-def adjoint_timing(cells=48, particles=65536, turns=512, every=32, warmup=2, reps=5):
+def adjoint_timing(cells=48, particles=65536, turns=512, every=32, warmup=2, reps=5, cpu_seconds=0.0):
     """Gradient of the amplitude loss with respect to all 96 quadrupole k1 and
     96 sextupole k2 of a 48-cell FODO ring (65 536 particles, sigma_x =
     sigma_y = 3 mm, 512 turns, checkpoints every 32 turns): time of the
@@ -514,7 +565,10 @@ def adjoint_timing(cells=48, particles=65536, turns=512, every=32, warmup=2, rep
     pass (the StableAperture kernel of the same ring and bunch, strengths
     compiled in), HIP events on the launch stream.  Each round times the
     three back to back; ``warmup`` rounds let the clock settle, the medians
-    of ``reps`` rounds are reported."""
+    of ``reps`` rounds are reported.  With ``cpu_seconds``: the oracle's
+    forward rate on a sample of the bunch, and the host time of a forward
+    pass and of the central-difference gradient (two passes per strength)
+    extrapolated from it."""
     import numpy as np
     import torch
     import ufo_amd as ufo
@@ -556,12 +610,31 @@ def adjoint_timing(cells=48, particles=65536, turns=512, every=32, warmup=2, rep
     f_ms, b_ms, p_ms = float(np.median(fw)), float(np.median(bw)), float(np.median(pf))
     pt = int(torch.clamp(lost.to(torch.int64) + 1, max=turns).sum())
     flop = pt * tr.kfwd.flops_per_turn
-    return {"workload": f"{cells}-cell FODO, {particles} particles, {turns} turns, checkpoints every {every}, "
-                        f"gradient w.r.t. {len(shared)} strengths",
-            "forward_ms": f_ms, "reverse_ms": b_ms, "gradient_over_forward": (f_ms + b_ms) / f_ms,
-            "plain_forward_ms": p_ms, "gradient_over_plain_forward": (f_ms + b_ms) / p_ms,
-            "forward_tflops": flop / (f_ms * 1e-3) / 1e12, "survivors": int((lost == turns).sum()),
-            "rounds": reps}
+    out = {"workload": f"{cells}-cell FODO, {particles} particles, {turns} turns, checkpoints every {every}, "
+                       f"gradient w.r.t. {len(shared)} strengths",
+           "forward_ms": f_ms, "reverse_ms": b_ms, "gradient_over_forward": (f_ms + b_ms) / f_ms,
+           "plain_forward_ms": p_ms, "gradient_over_plain_forward": (f_ms + b_ms) / p_ms,
+           "forward_tflops": flop / (f_ms * 1e-3) / 1e12, "survivors": int((lost == turns).sum()),
+           "rounds": reps}
+    if cpu_seconds:
+        # the host has no adjoint: its route to the gradient is central
+        # differences, two forward passes per strength
+        from oracle import runner
+        E = sa.kernel.elements
+
+        def run_rows(idx):
+            return _executed(runner.stable_aperture(ring, bunch[idx], flags=ufo.KICK | ufo.DOUBLE_PRECISION,
+                                                    turns=turns, parameters=["x", "px", "y", "py", "dp"],
+                                                    mode="bench_fast"), turns)
+
+        rate, m, secs = _cpu_sample_rate(run_rows, particles, E, cpu_seconds)
+        fwd_s = pt * E / rate
+        out["cpu_baseline"] = {"pet_per_s": rate, "kind": "port", "build": "bench_fast",
+                               "sample": f"{m} of the {particles} particles, {turns} turns, {secs:.1f} s",
+                               "forward_seconds_extrapolated": fwd_s,
+                               "central_difference_gradient_seconds_extrapolated": 2 * len(shared) * fwd_s,
+                               "gpu_gradient_speedup": 2 * len(shared) * fwd_s / ((f_ms + b_ms) * 1e-3)}
+    return out
 
 
 def measured_traffic(args):
@@ -607,6 +680,32 @@ def _host_record():
         except OSError:
             pass
     return rec
+
+
+def _cpu_sample_rate(run_rows, n, E, budget_s):
+    """Host rate of the oracle over an evenly strided sample of rows
+    0..n-1 sized to ~budget_s seconds: (PET/s, sample size, seconds).
+    ``run_rows(indices)`` runs the oracle (gcc fast-math build, OpenMP over
+    the host threads) on those rows and returns their executed
+    particle-turns."""
+    import numpy as np
+    rows = np.arange(n)
+    probe = rows[:: max(1, n // 128)][:128]
+    run_rows(probe)                              # builds (cached) and warms the library
+    t = time.perf_counter()
+    run_rows(probe)
+    per_row = (time.perf_counter() - t) / len(probe)
+    m = int(min(n, max(128, budget_s / max(per_row, 1e-9))))
+    sample = rows[:: max(1, n // m)][:m]
+    t = time.perf_counter()
+    pt = run_rows(sample)
+    dt = time.perf_counter() - t
+    return pt * E / dt, len(sample), dt
+
+
+def _executed(lost, turns):
+    import numpy as np
+    return int(np.minimum(np.asarray(lost).astype(np.int64) + 1, turns).sum())
 
 
 def cpu_baseline(ring, flags, pool, turns, E, budget_s, gpu_lost=None):
