@@ -298,10 +298,10 @@ class HostAdjoint:
     (final state [n, 4], lost [n], saved), ``backward(lam, saved)`` ->
     (dL/dx0 [n, 4], dL/dtheta)."""
 
-    def __init__(self, line, flags, turns, coords, lattice_parameters, every=8):
+    def __init__(self, line, flags, turns, coords, lattice_parameters, every=8, dp=0.0):
         from ufo_amd import adjoint
         self.kfwd, self.kbwd, self.ktab, self.layout, self.ncheck = adjoint.adjoint_kernels(
-            line, flags, turns, coords, lattice_parameters, every=every)
+            line, flags, turns, coords, lattice_parameters, dp=dp, every=every)
         self.turns, self.npar = turns, max(1, len(lattice_parameters))
         # one translation unit, each kernel's source in a namespace of its
         # own (the sources repeat their helper functions)
@@ -346,7 +346,10 @@ class HostAdjoint:
         gpart = np.full((nb, self.npar), np.nan)
         self.lib.host_bwd(ctypes.c_int(nb), ctypes.c_int(block), _ptr(x0), _ptr(ct), _ptr(ckpt), _ptr(lam),
                           _ptr(lost), _ptr(gx0), _ptr(gpart), ctypes.c_int(n))
-        return gx0, gpart.sum(axis=0)
+        g = gpart.sum(axis=0)
+        scale = np.asarray(self.layout["gscale"], dtype=np.float64)
+        g[:scale.size] *= scale     # the factors the kernel leaves to the reduction
+        return gx0, g
 
 
 _BADJ_DRIVER = r"""

@@ -131,3 +131,52 @@ def test_batched_family_gradients_per_seed():
     err = np.abs(gseed - fd) / np.abs(fd).max()
     print(f"per-seed family gradients vs central differences: max rel {err.max():.2e}")
     assert err.max() < 1e-6
+
+
+@pytest.mark.parametrize("mode", ["6d", "fived"])
+def test_every_cell_strength_rotated_loop(mode):
+    """A ring of 6 FODO cells with every quadrupole k1 and sextupole k2 a
+    parameter of its own (the per-cell strengths of the 48-cell workload):
+    the reverse pass loops over the cells with the loop body rotated past
+    the leading drift (the drift closing one cell and the one opening the
+    next are one drift), runs on scaled momenta p (1 + dp) so the kicks use
+    the table's strengths without a product with 1/(1+dp), and leaves the
+    constant slice factors to the reduction (each parameter's total is
+    scaled once)."""
+    from ufo_amd import adjoint
+    cells = 6
+    ring = lattices.fodo_ring(cells, distinct=True)
+    shared = ([(f"QF{i}", "k1") for i in range(cells)] + [(f"QD{i}", "k1") for i in range(cells)]
+              + [(f"SF{i}", "k2") for i in range(cells)] + [(f"SD{i}", "k2") for i in range(cells)])
+    theta = np.array([1.2] * cells + [-1.2] * cells + [0.8] * cells + [-1.4] * cells)
+    theta = theta * (1.0 + 0.01 * np.arange(len(theta)) / len(theta))
+    bunch = lattices.gaussian_bunch(40, seed=4, sx=3e-3, sy=3e-3)
+    flags = ufo.KICK | ufo.DOUBLE_PRECISION
+    coords = COORDS
+    if mode == "fived":
+        flags |= ufo.FIVED
+        coords = ["x", "px", "y", "py"]
+        bunch = bunch[:, :4]
+    fk, bk, tk, layout, _ = adjoint.adjoint_kernels(ring, flags, 24, coords, shared, dp=1e-3, every=8)
+    assert "for (int rep = 4; rep >= 0; --rep)" in bk.src        # 6 cells: 5 in the loop
+    assert sorted(set(layout["gscale"])) != [1.0]
+    loop = bk.src.split("for (int rep = 4; rep >= 0; --rep)")[1]
+    assert "px = c[np] / oodppo" in bk.src and "* oodppo" not in loop
+    if mode == "fived":
+        adj = HostAdjoint(ring, flags, 24, coords, shared, every=8, dp=1e-3)
+        assert adj.layout["gscale"] == layout["gscale"]
+        rng = np.random.default_rng(7)
+        w = rng.normal(size=(len(bunch), 4))
+        xf, lost, saved = adj.forward(bunch, theta)
+        assert np.all(lost == 24)
+        gx0, g = adj.backward(w, saved)
+        h = 1e-6 * np.abs(theta)
+        eye = np.eye(len(theta))
+        fd = np.array([(np.sum(w * adj.forward(bunch, theta + h[i] * eye[i])[0])
+                        - np.sum(w * adj.forward(bunch, theta - h[i] * eye[i])[0])) / (2 * h[i])
+                       for i in range(len(theta))])
+        err = np.abs(g - fd) / np.max(np.abs(fd))
+        print(f"FIVED dL/dtheta vs central differences: max rel {err.max():.2e}")
+        assert err.max() < 1e-6
+    else:
+        _check(ring, flags, shared, theta, bunch)

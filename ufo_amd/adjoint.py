@@ -83,9 +83,11 @@ def emit_reverse(op, env: Env, out: List[str], g: _Grad):
         L = f(op.L)
         if is_zero(L):
             return
-        v = env.val(L)
+        # with scaled momenta p' = p (1 + dp) the drift is x += p' L/(1+dp)
+        coef = op.L * ir.OODPPO if env.scaled_momenta else op.L
+        v = env.val(coef)
         out.append(f"x = fma(-px, {v}, x); y = fma(-py, {v}, y);")
-        _add_grad(env, g, op.L, [("lx", "px"), ("ly", "py")], out)
+        _add_grad(env, g, coef, [("lx", "px"), ("ly", "py")], out)
         out.append(f"lpx = fma(lx, {v}, lpx); lpy = fma(ly, {v}, lpy);")
     elif isinstance(op, ir.ExactDrift):
         L = env.val(op.L)
@@ -99,6 +101,11 @@ def emit_reverse(op, env: Env, out: List[str], g: _Grad):
         if ir.Expr.of(op.L).params():
             raise NotImplementedError("gradient with respect to an exact-drift length")
     elif isinstance(op, ir.Kick):
+        if env.scaled_momenta:
+            # p' = p (1 + dp) takes the kick's 1/(1+dp) factor
+            if not op.chroma:
+                raise ValueError("scaled momenta need every kick scaled by 1/(1+dp)")
+            op = ir.Kick(op.kn, op.ks, False)
         emit_kick_reverse(op, env, out, g)
     elif isinstance(op, ir.Weak):
         w = f(op.w)
@@ -404,10 +411,94 @@ def merge_drifts(ops, f):
     return out
 
 
+def _chi_factor(e: ir.Expr):
+    """A folded derivative as ('const', c) for a constant c, ('chi', c) for
+    c * oodppo, or None for anything else (per-particle, table-dependent)."""
+    t = e.t
+    if t[0] == "c":
+        return ("const", t[1])
+    if t == ("v", "oodppo"):
+        return ("chi", 1.0)
+    if t[0] == "neg":
+        a = _chi_factor(t[1])
+        return None if a is None else (a[0], -a[1])
+    if t[0] in ("*", "/"):
+        a, b = _chi_factor(t[1]), _chi_factor(t[2])
+        if a is None or b is None:
+            return None
+        if t[0] == "*":
+            if a[0] == "chi" and b[0] == "chi":
+                return None
+            return ("chi" if "chi" in (a[0], b[0]) else "const", a[1] * b[1])
+        if b[0] != "const" or b[1] == 0.0:
+            return None
+        return (a[0], a[1] / b[1])
+    return None
+
+
+def _rotation(ops, head, p, f):
+    """Number of leading ops of the period (parameter-free drifts and the
+    no-op shifts and weak terms that ``merge_drifts`` drops) to move to the
+    end of the loop body, so that the drift ending one repetition and the
+    drift starting the next fall in one segment and merge; 0 when that
+    merges nothing."""
+    period = ops[head:head + p]
+    k, drift = 0, False
+    for op in period:
+        if isinstance(op, ir.Drift) and not ir.Expr.of(op.L).params():
+            drift = True
+        elif not ((isinstance(op, ir.Shift) and is_zero(f(op.dx)) and is_zero(f(op.dy)))
+                  or (isinstance(op, ir.Weak) and is_zero(f(op.w)))):
+            break
+        k += 1
+    if not drift or k >= p:
+        return 0
+    rest = merge_drifts(period[k:], f)
+    return k if rest and isinstance(rest[-1], ir.Drift) else 0
+
+
+def _scalable(ops, f) -> bool:
+    """True when the turn holds only paraxial drifts, kicks scaled by
+    1/(1+dp) and apertures once no-op shifts and weak terms are dropped: the
+    ops that the reverse pass can run on scaled momenta."""
+    for op in merge_drifts(ops, f):
+        if isinstance(op, (ir.Drift, ir.Aperture)) or (isinstance(op, ir.Kick) and op.chroma):
+            continue
+        return False
+    return True
+
+
 def _reverse_layout(ops, env: Env, ntheta: int):
     """Reverse pass of one turn: the ops backwards, the repeated period as a
-    descending loop; gradient registers flushed to LDS per segment."""
+    descending loop; gradient registers flushed to LDS per segment.
+
+    The loop body is the period rotated past its leading drift, so the
+    drift closing one repetition and the one opening the next are a single
+    drift of the reverse pass (the last repetition's body and the first's
+    leading drift run outside the loop).
+
+    Gradient scaling: the flush of a sum multiplies it by the derivative of
+    its coefficient.  When every derivative is a constant times 1/(1+dp)
+    (kicks in 6D), lambda is carried multiplied by oodppo instead (the
+    recursion lambda <- J^T lambda is linear, so every sum carries the
+    factor) and divided by it again for dL/dx0; when every derivative is
+    then a constant, and one constant serves all sums of a parameter, that
+    constant is applied to the parameter's total after the reduction
+    (``gscale``) and the flush adds the bare sum.
+
+    Scaled momenta: when the turn is drifts, kicks scaled by 1/(1+dp) and
+    apertures (``_scalable``), the reverse pass carries p' = p (1 + dp) and
+    lambda_p' = lambda_p / (1 + dp) instead of p and lambda_p.  The kicks
+    then use the table's strengths directly (no per-magnet product with
+    1/(1+dp) in the loop), each drift's L/(1+dp) is a per-particle
+    invariant, and the strength derivatives are constants.  Returns (lines,
+    period, {"lam_scaled": bool, "scaled_momenta": bool, "gscale": [ntheta]})."""
     head, p, reps, stride = codegen.find_period(ops, True) if len(ops) > 8 else (0, 0, 0, 0)
+    rot = _rotation(ops, head, p, env.f) if reps >= 3 else 0
+    lo = head + rot                        # first op of the loop body
+    n = reps - 1 if rot else reps          # repetitions inside the loop
+    flushes = []     # (placeholder, parameters it adds to, index text, [(sum, plain text, factor)])
+    env.scaled_momenta = _scalable(ops, env.f)
 
     def segment(seg_ops, lv=None):
         g = _Grad()
@@ -417,33 +508,82 @@ def _reverse_layout(ops, env: Env, ntheta: int):
         decl = [f"ufloat {r} = {env.lit(0.0)};" for r in g.sums.values()]
         flush = []
         for i in g.params():
-            terms = [(r, env.val(dd)) for (ii, dd), r in g.sums.items() if ii == i]
-            acc_ = f"{terms[0][0]} * {terms[0][1]}"
-            for r, dv in terms[1:]:
-                acc_ = f"fma({r}, {dv}, {acc_})"
+            terms = [(r, env.val(dd), _chi_factor(env.f(dd))) for (ii, dd), r in g.sums.items() if ii == i]
             j = i - env.shared_offset
             idx = f"({j} + {lv} * {stride})" if (lv and stride) else f"{j}"
-            flush.append(f"unsafeAtomicAdd(&acc[{idx} * {ACCW} + slot], {acc_});")
+            tag = f"@FLUSH{len(flushes)}@"
+            covered = sorted({j + r * int(stride) for r in range(n)}) if lv else [j]
+            flushes.append((tag, covered, idx, terms))
+            flush.append(tag)
         return ["{"] + ["    " + s for s in decl + body + flush] + ["}"]
 
     out: List[str] = []
     if reps >= 2:
-        out += segment(ops[head + p * reps:])
+        out += segment(ops[lo + p * n:])
         env.stride = ("rep", stride)
         try:
-            inner = segment(ops[head:head + p], "rep")
+            inner = segment(ops[lo:lo + p], "rep")
         finally:
             env.stride = None
-        pre, top, end = codegen.prefetch_lines(env, str(reps - 1), "(rep > 0 ? rep - 1 : 0)")
+        pre, top, end = codegen.prefetch_lines(env, str(n - 1), "(rep > 0 ? rep - 1 : 0)")
         out += pre
         out.append(codegen.loop_pragma(env))
-        out.append(f"for (int rep = {reps - 1}; rep >= 0; --rep) {{")
+        out.append(f"for (int rep = {n - 1}; rep >= 0; --rep) {{")
         out += ["    " + s for s in top + inner + end]
         out.append("}")
-        out += segment(ops[:head])
+        out += segment(ops[:lo])
     else:
         out += segment(ops)
-    return out, (head, p, reps, stride)
+
+    kinds = {t[2][0] if t[2] else None for _, _, _, terms in flushes for t in terms}
+    factored = None not in kinds and kinds <= {"const"} | ({"chi"} if kinds == {"chi"} else set())
+    gscale = [1.0] * ntheta
+    bare = set()      # flushes that add the bare sum
+    if factored:
+        # a parameter is scaled after the reduction when every flush adding
+        # to it is a single sum with one common factor, and every other
+        # parameter those flushes add to is scaled likewise
+        per = {}
+        for k, (_, cov, _, terms) in enumerate(flushes):
+            for j in cov:
+                per.setdefault(j, []).append(k)
+        ok = {j for j, ks in per.items()
+              if all(len(flushes[k][3]) == 1 for k in ks) and len({flushes[k][3][0][2][1] for k in ks}) == 1}
+        changed = True
+        while changed:
+            changed = False
+            for j in list(ok):
+                if any(c not in ok for k in per[j] for c in flushes[k][1]):
+                    ok.discard(j)
+                    changed = True
+        for j in ok:
+            gscale[j] = flushes[per[j][0]][3][0][2][1]
+        bare = {k for k, (_, cov, _, terms) in enumerate(flushes)
+                if all(j in ok for j in cov) or (len(terms) == 1 and terms[0][2][1] == 1.0)}
+    text = {}
+    for k, (tag, cov, idx, terms) in enumerate(flushes):
+        if k in bare:
+            acc_ = terms[0][0]
+        elif factored:
+            acc_ = f"{terms[0][0]} * {env.lit(terms[0][2][1])}"
+            for r, _, fc in terms[1:]:
+                acc_ = f"fma({r}, {env.lit(fc[1])}, {acc_})"
+        else:
+            acc_ = f"{terms[0][0]} * {terms[0][1]}"
+            for r, dv, _ in terms[1:]:
+                acc_ = f"fma({r}, {dv}, {acc_})"
+        text[tag] = f"unsafeAtomicAdd(&acc[{idx} * {ACCW} + slot], {acc_});"
+    out = [_fill_flush(s, text) for s in out]
+    info = {"lam_scaled": factored and kinds == {"chi"}, "scaled_momenta": env.scaled_momenta, "gscale": gscale}
+    env.scaled_momenta = False
+    return out, (head, p, reps, stride), info
+
+
+def _fill_flush(line: str, text: dict) -> str:
+    if "@FLUSH" not in line:
+        return line
+    key = line.strip()
+    return line.replace(key, text[key])
 
 
 def _table_kernel(env: Env, flat, period, reps, stride, F, S, name="ufo_adj_table"):
@@ -518,12 +658,12 @@ def adjoint_kernels(line, flags, turns, coords, lattice_parameters, dp=0.0, ever
     envf = make_env()
     fbody, period = codegen._layout(ops, envf)
     envb = make_env()
-    rbody, rperiod = _reverse_layout(ops, envb, nth)
+    rbody, rperiod, rinfo = _reverse_layout(ops, envb, nth)
     head, p, reps, stride = rperiod
     F, S = len(table), len(slots)
     reps = int(reps) if reps >= 2 and S else 0
     ns = max(1, F + reps * S)
-    layout = {"flat": [e for e, _ in table], "period": sorted(slots, key=slots.get), "reps": reps,
+    layout = {"flat": [e for e, _ in table], "period": sorted(slots, key=slots.get), "reps": reps, "gscale": rinfo["gscale"][:len(shared)],
               "stride": int(stride), "F": F, "S": S, "nslots": ns}
 
     def fill(text):
@@ -561,6 +701,12 @@ ufo_adj_fwd(const ufloat *__restrict__ pool, const ufloat *__restrict__ ct, uflo
                         period=period)
 
     load = codegen._load_block(envb, src, P, "row", "        ")
+    # lambda carried times oodppo when every gradient factor holds it
+    ls, us = (" * oodppo", " / oodppo") if rinfo["lam_scaled"] else ("", "")
+    # scaled momenta: p' = p (1 + dp), lambda_p' = lambda_p / (1 + dp)
+    sm = rinfo["scaled_momenta"]
+    lp, up = (ls + " * oodppo", us + " / oodppo") if sm else (ls, us)
+    pin = " / oodppo" if sm else ""
     bwd = pre + fill(f"""
 extern "C" __global__ void __launch_bounds__({block})
 ufo_adj_bwd(const ufloat *__restrict__ pool, const ufloat *__restrict__ ct, const ufloat *__restrict__ ckpt,
@@ -577,19 +723,19 @@ ufo_adj_bwd(const ufloat *__restrict__ pool, const ufloat *__restrict__ ct, cons
         const size_t np = (size_t)particles;
         const ufloat *__restrict__ row = pool + (size_t)idx * {P};
 {chr(10).join(load)}
-        ufloat lx = lam[(size_t)idx * 4], lpx = lam[(size_t)idx * 4 + 1];
-        ufloat ly = lam[(size_t)idx * 4 + 2], lpy = lam[(size_t)idx * 4 + 3];
+        ufloat lx = lam[(size_t)idx * 4]{ls}, lpx = lam[(size_t)idx * 4 + 1]{lp};
+        ufloat ly = lam[(size_t)idx * 4 + 2]{ls}, lpy = lam[(size_t)idx * 4 + 3]{lp};
         for (int ck = {nck - 1}; ck >= 1; --ck) {{
             const int t_end = (ck * {every} < {turns}) ? ck * {every} : {turns};
             const int t_begin = (ck - 1) * {every};
             const ufloat *c = ckpt + (size_t)ck * 4 * np + idx;
-            x = c[0]; px = c[np]; y = c[2 * np]; py = c[3 * np];
+            x = c[0]; px = c[np]{pin}; y = c[2 * np]; py = c[3 * np]{pin};
             for (int turn = t_end - 1; turn >= t_begin; --turn) {{
 {chr(10).join('                ' + s_ for s_ in rbody)}
             }}
         }}
         ufloat *o = gx0 + (size_t)idx * 4;
-        o[0] = lx; o[1] = lpx; o[2] = ly; o[3] = lpy;
+        o[0] = lx{us}; o[1] = lpx{up}; o[2] = ly{us}; o[3] = lpy{up};
     }}
     __syncthreads();
     for (int j = threadIdx.x; j < {nth}; j += {block}) {{
@@ -639,6 +785,10 @@ class AdjointTracker(Simulation):
         self.kernel = self.kfwd
         self._mods = None
         self._saved = None
+        # per-parameter factors the reverse kernel leaves to the reduction
+        gs = self.layout["gscale"]
+        self._gscale = (torch.tensor(gs, dtype=torch.float64, device=self.device)
+                        if any(v != 1.0 for v in gs) else None)
 
     def modules(self):
         if self._mods is None:
@@ -713,7 +863,10 @@ class AdjointTracker(Simulation):
             bwd.launch(self.kbwd.name, nb, block, [pool, th, ckpt, lam, lost, gx0, gpart, n])
         else:
             gpart.zero_()
-        return gx0, gpart.sum(dim=0)[:len(self.lattice_parameters)]
+        g = gpart.sum(dim=0)[:len(self.lattice_parameters)]
+        if self._gscale is not None:
+            g = g * self._gscale
+        return gx0, g
 
 
 class _TrackFunction(torch.autograd.Function):

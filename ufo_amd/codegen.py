@@ -88,6 +88,7 @@ class Env:
     products: Dict[tuple, str] = field(default_factory=dict)  # prefetch mode: (slot, variable) -> register
     remap_loops: bool = False        # table mode: loop over repetitions whose parameters map column to column
     unroll: int = 1                  # period-loop unroll factor (1: the loop stays rolled)
+    scaled_momenta: bool = False     # reverse pass: px, py carried times (1 + dp), lambda_px, lambda_py over it
 
     def f(self, e) -> Expr:
         return fold(e, self.fp32, self.consts)
