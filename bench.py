@@ -193,7 +193,11 @@ def lossy_scan_timing(nx, ny, turns, reps, lossless_tflops, sext_scale=4.0):
     particle-turns.  ``utilisation_vs_lossless`` is the achieved FP64 rate
     over that of the lossless scan (whose issued and canonical FLOPs are
     equal): the fraction of the FP64 issue the lossy launch turns into
-    useful particle-turns (loss, refill and the tail of the launch)."""
+    useful particle-turns (loss, refill and the tail of the launch).  Lanes
+    claim particles in StableAperture's default order (runs of 64 by
+    increasing amplitude, the sort included in every timed launch);
+    ``kernel_ms_pool_order`` times the same launches claiming in pool
+    order."""
     import numpy as np
     import torch
     import ufo_amd as ufo
@@ -207,14 +211,25 @@ def lossy_scan_timing(nx, ny, turns, reps, lossless_tflops, sext_scale=4.0):
     sa.pool_device.copy_(sa._pool_host)
     sa.launch()
     stream = torch.cuda.current_stream()
-    ms = []
-    for _ in range(max(1, reps)):
-        e = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
-        e[0].record(stream)
-        sa.launch()
-        e[1].record(stream)
-        torch.cuda.synchronize()
-        ms.append(e[0].elapsed_time(e[1]))
+
+    def timed():
+        ms = []
+        for _ in range(max(1, reps)):
+            e = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+            e[0].record(stream)
+            sa.launch()
+            e[1].record(stream)
+            torch.cuda.synchronize()
+            ms.append(e[0].elapsed_time(e[1]))
+        return ms
+
+    # the same launches with lanes claiming particles in pool order, for
+    # comparison with the default claim order
+    order = sa.claim_order
+    sa.claim_order = None
+    ms_pool = timed()
+    sa.claim_order = order
+    ms = timed()
     lost = sa.lost_device.cpu().numpy().view(np.uint32)
     executed = np.minimum(lost.astype(np.int64) + 1, turns)
     pt = int(executed.sum())
@@ -227,7 +242,8 @@ def lossy_scan_timing(nx, ny, turns, reps, lossless_tflops, sext_scale=4.0):
             "executed_particle_turns": pt, "executed_fraction": pt / (len(pool) * turns),
             "loss_turn_quantiles_10_50_90": [float(q) for q in np.percentile(lost[lost < turns], [10, 50, 90])],
             "tflops": tflops, "frac_fp64_peak": tflops / PEAK_FP64_TFLOPS,
-            "utilisation_vs_lossless": tflops / lossless_tflops, "lanes": sa.lanes}
+            "utilisation_vs_lossless": tflops / lossless_tflops, "lanes": sa.lanes,
+            "claim_order": sa.claim_order, "kernel_ms_pool_order": round(float(np.mean(ms_pool)), 3)}
 
 
 def fodo_track_timing(particles=4096, turns=1000, reps=3, cpu=True):

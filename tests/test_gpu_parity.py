@@ -469,7 +469,10 @@ def test_optics_chromaticity_by_dp_parameter():
 
 def test_sa_threads_and_refill():
     """Results do not depend on the number of resident lanes (persistent
-    refill) — run with 64 lanes and with the full device."""
+    refill) nor on the order in which lanes claim particles — run with 64
+    lanes and with the full device, claiming by amplitude (the default) and
+    in pool order."""
+    import torch
     ring = lattices.fodo_ring(16)
     flags = ufo.KICK | ufo.DOUBLE_PRECISION | ufo.FIVED
     pool = lattices.da_grid(40, 40, xlim=0.05, ymax=0.05)
@@ -477,14 +480,19 @@ def test_sa_threads_and_refill():
     sa.parameters[:] = pool
     sa.run()
     full = sa.lost.copy()
-    sa._pool_dev.copy_(sa._pool_host)
-    sa.launch(lanes=64)
-    sa._lost_host.copy_(sa._lost_dev)
-    import torch
-    torch.cuda.synchronize()
-    assert np.array_equal(sa.lost, full)
+    order = sa.order_device()
+    assert order is not None and sorted(order.cpu().tolist()) == list(range(len(pool)))
+    for claim in ("amplitude", None):
+        sa.claim_order = claim
+        sa._pool_dev.copy_(sa._pool_host)
+        sa._lost_dev.fill_(-1)
+        sa.launch(lanes=64)
+        sa._lost_host.copy_(sa._lost_dev)
+        torch.cuda.synchronize()
+        assert np.array_equal(sa.lost, full), claim
     ref = runner.stable_aperture(ring, pool, flags=flags, turns=300, parameters=["x", "y"])
     assert np.array_equal(full, ref)
+    assert 0 < np.sum(full < 300) < len(pool)
 
 
 @pytest.mark.parametrize("double", [False, True])

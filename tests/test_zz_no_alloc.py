@@ -25,7 +25,7 @@ def _noalloc(sa, lanes):
     grid, block = sa.launch_config(lanes)
     print(f"[no_alloc] particles={sa.particles} lanes={grid * block}", flush=True)
     sa.module().launch_noalloc(sa.kernel.name, grid, block,
-                               [sa.pool_device, lost, None, sa.particles, grid * block],
+                               [sa.pool_device, lost, None, sa.particles, grid * block, None],
                                scratch_arg=2, scratch_bytes=4)
     torch.cuda.synchronize()
     return lost.cpu().numpy().view(np.uint32)
@@ -59,8 +59,8 @@ def test_noalloc_rejects_oversized_scratch():
                             particles=64, parameters=["x", "y"])
     lost = torch.zeros((64,), dtype=torch.int32, device=sa.device)
     with pytest.raises(runtime.UfoError, match="scratch of 1048577 bytes"):
-        sa.module().launch_noalloc(sa.kernel.name, 1, 64, [sa.pool_device, lost, None, 64, 64],
+        sa.module().launch_noalloc(sa.kernel.name, 1, 64, [sa.pool_device, lost, None, 64, 64, None],
                                    scratch_arg=2, scratch_bytes=(1 << 20) + 1)
-    with pytest.raises(runtime.UfoError, match="outside the 5 kernel arguments"):
-        sa.module().launch_noalloc(sa.kernel.name, 1, 64, [sa.pool_device, lost, None, 64, 64],
-                                   scratch_arg=5, scratch_bytes=4)
+    with pytest.raises(runtime.UfoError, match="outside the 6 kernel arguments"):
+        sa.module().launch_noalloc(sa.kernel.name, 1, 64, [sa.pool_device, lost, None, 64, 64, None],
+                                   scratch_arg=6, scratch_bytes=4)

@@ -992,18 +992,21 @@ def sa_kernel(line, flags, turns, parameters, dp, block=256, name="ufo_sa") -> K
     code += f"""
 extern "C" __global__ void __launch_bounds__({block})
 {name}(const ufloat *__restrict__ pool, unsigned int *__restrict__ lost,
-       int *__restrict__ counter, int particles, int first)
+       int *__restrict__ counter, int particles, int first, const int *__restrict__ order)
 {{
-    // lanes start on particles [0, first); further particles are claimed
-    // from counter (zeroed by the host before the launch)
+    // lanes start on claims [0, first); further claims come from counter
+    // (zeroed by the host before the launch).  Claim c tracks particle
+    // order[c] (a permutation of the particles), or particle c without order.
     {_decls(env)}
-    int idx = blockIdx.x * {block} + threadIdx.x;
+    int claim = blockIdx.x * {block} + threadIdx.x;
+    int idx = 0;
     int turn = 0;
     bool need = true;
     const ufloat *__restrict__ row = pool;
     for (;;) {{
         if (need) {{
-            if (idx >= particles) break;
+            if (claim >= particles) break;
+            idx = order ? order[claim] : claim;
             row = pool + (size_t)idx * {P};
 {chr(10).join(load)}
             turn = 0;
@@ -1021,7 +1024,7 @@ extern "C" __global__ void __launch_bounds__({block})
             int base = 0;
             if (lane == leader) base = atomicAdd(counter, __popcll(m));
             base = __shfl(base, leader);
-            idx = first + base + rank;
+            claim = first + base + rank;
             need = true;
         }} else {{
             ++turn;

@@ -1,6 +1,8 @@
 """StableAperture: turns survived by each particle (ufo/sa.py:18-145)."""
 from __future__ import annotations
 
+import ctypes
+
 import numpy as np
 import torch
 
@@ -62,6 +64,46 @@ class StableAperture(Simulation):
     # particle; one wave per SIMD leaves the pipe waiting on latency.
     WAVES_PER_SIMD = 2
 
+    # Order in which lanes claim particles: "amplitude" claims runs of
+    # CLAIM_RUN consecutive particles by increasing mean normalised
+    # amplitude (the runs likely to survive longest first), None in pool
+    # order.  Claiming the long-lived particles first leaves the short-lived
+    # ones for the end of the launch, where they fill the gaps of lanes still
+    # on their last particle; whole runs keep a wave's refill reading and
+    # writing consecutive entries.  Results are per particle and do not
+    # depend on the order.
+    claim_order = "amplitude"
+    CLAIM_RUN = 64
+
+    _AMPLITUDE_COLUMNS = ("x", "px", "y", "py", "dp")
+
+    def order_device(self):
+        """Claim order of the device pool (int32 permutation on the device),
+        or None: the pool cut into runs of ``CLAIM_RUN`` consecutive
+        particles, the runs sorted by the mean over their particles of the
+        sum over x, px, y, py, dp (the columns the pool has) of (value /
+        largest |value| of that column)^2 (NaN values count as 0)."""
+        if self.claim_order is None or self.particles <= self.CLAIM_RUN:
+            return None
+        if self.claim_order != "amplitude":
+            raise ValueError(f"claim_order {self.claim_order!r}: 'amplitude' or None")
+        cols = [self.parameter_names.index(c) for c in self._AMPLITUDE_COLUMNS if c in self.parameter_names]
+        if not cols:
+            return None
+        v = self._pool_dev[:, cols].nan_to_num(nan=0.0, posinf=0.0, neginf=0.0)
+        s = v.abs().amax(dim=0)
+        s = torch.where(s > 0, s, torch.ones_like(s))
+        key = (v / s).square().sum(dim=1)
+        R, n = self.CLAIM_RUN, self.particles
+        runs = (n + R - 1) // R
+        pad = torch.zeros(runs * R, dtype=key.dtype, device=key.device)
+        pad[:n] = key
+        mean = pad.view(runs, R).sum(dim=1)
+        mean[-1] = mean[-1] * R / (n - (runs - 1) * R)          # the last run may be short
+        first = torch.argsort(mean, stable=True).to(torch.int32) * R
+        order = (first[:, None] + torch.arange(R, dtype=torch.int32, device=key.device)[None, :]).reshape(-1)
+        return order[order < n].contiguous()
+
     def launch_config(self, lanes=None):
         block = self.kernel.block
         if lanes is None:
@@ -81,9 +123,11 @@ class StableAperture(Simulation):
             return
         grid, block = self.launch_config(lanes)
         self.lanes = grid * block
+        order = self.order_device()
         self._counter.zero_()
         self.module().launch(self.kernel.name, grid, block,
-                             [self._pool_dev, self._lost_dev, self._counter, self.particles, grid * block])
+                             [self._pool_dev, self._lost_dev, self._counter, self.particles, grid * block,
+                              order if order is not None else ctypes.c_void_p(0)])
 
     def run(self, threads=1):
         """Copy ``parameters`` to the device, track, copy ``lost`` back."""
