@@ -65,7 +65,7 @@ class StableAperture(Simulation):
     WAVES_PER_SIMD = 2
 
     # Order in which lanes claim particles: "amplitude" claims runs of
-    # CLAIM_RUN consecutive particles by increasing mean normalised
+    # CLAIM_RUN consecutive particles by increasing summed normalised
     # amplitude (the runs likely to survive longest first), None in pool
     # order.  Claiming the long-lived particles first leaves the short-lived
     # ones for the end of the launch, where they fill the gaps of lanes still
@@ -80,29 +80,28 @@ class StableAperture(Simulation):
     def order_device(self):
         """Claim order of the device pool (int32 permutation on the device),
         or None: the pool cut into runs of ``CLAIM_RUN`` consecutive
-        particles, the runs sorted by the mean over their particles of the
-        sum over x, px, y, py, dp (the columns the pool has) of (value /
-        largest |value| of that column)^2 (NaN values count as 0)."""
-        if self.claim_order is None or self.particles <= self.CLAIM_RUN:
+        particles, the full runs sorted by the sum over their particles of
+        the sum over x, px, y, py, dp (the columns the pool has) of (value /
+        largest |value| of that column)^2 (NaN values count as 0), a short
+        last run claimed last.  Device work only: no host synchronisation."""
+        if self.claim_order is None:
             return None
         if self.claim_order != "amplitude":
             raise ValueError(f"claim_order {self.claim_order!r}: 'amplitude' or None")
+        R, n = self.CLAIM_RUN, self.particles
+        runs = n // R
         cols = [self.parameter_names.index(c) for c in self._AMPLITUDE_COLUMNS if c in self.parameter_names]
-        if not cols:
+        if runs < 2 or not cols:
             return None
-        v = self._pool_dev[:, cols].nan_to_num(nan=0.0, posinf=0.0, neginf=0.0)
+        v = self._pool_dev[:runs * R, cols].nan_to_num(nan=0.0, posinf=0.0, neginf=0.0)
         s = v.abs().amax(dim=0)
         s = torch.where(s > 0, s, torch.ones_like(s))
-        key = (v / s).square().sum(dim=1)
-        R, n = self.CLAIM_RUN, self.particles
-        runs = (n + R - 1) // R
-        pad = torch.zeros(runs * R, dtype=key.dtype, device=key.device)
-        pad[:n] = key
-        mean = pad.view(runs, R).sum(dim=1)
-        mean[-1] = mean[-1] * R / (n - (runs - 1) * R)          # the last run may be short
-        first = torch.argsort(mean, stable=True).to(torch.int32) * R
-        order = (first[:, None] + torch.arange(R, dtype=torch.int32, device=key.device)[None, :]).reshape(-1)
-        return order[order < n].contiguous()
+        key = (v / s).square().view(runs, R * len(cols)).sum(dim=1)
+        lane = torch.arange(R, dtype=torch.int32, device=v.device)
+        order = (torch.argsort(key, stable=True).to(torch.int32)[:, None] * R + lane[None, :]).view(-1)
+        if runs * R < n:
+            order = torch.cat([order, torch.arange(runs * R, n, dtype=torch.int32, device=v.device)])
+        return order
 
     def launch_config(self, lanes=None):
         block = self.kernel.block
