@@ -180,3 +180,26 @@ def test_every_cell_strength_rotated_loop(mode):
         assert err.max() < 1e-6
     else:
         _check(ring, flags, shared, theta, bunch)
+
+
+def test_family_strength_over_magnets_of_different_lengths():
+    """One sextupole family strength bound (by label) to magnets of two
+    lengths in every cell, plus per-cell quadrupole strengths: the family's
+    sums carry different constant factors (k2 L / (2 n) per slice), so the
+    reverse pass multiplies them at each flush while the quadrupoles' totals
+    are scaled after the reduction."""
+    from ufo_amd import adjoint
+    cells = 4
+    out = []
+    for i in range(cells):
+        out += [ufo.Quadrupole(f"QF{i}", length=0.5, k1=1.2), ufo.Sextupole("SF", length=0.2, k2=6.0),
+                ufo.Drift(f"DA{i}", length=2.0), ufo.Quadrupole(f"QD{i}", length=0.5, k1=-1.2),
+                ufo.Sextupole("SF", length=0.3, k2=6.0), ufo.Drift(f"DB{i}", length=2.0)]
+    ring = ufo.Line("R", out)
+    shared = [(f"QF{i}", "k1") for i in range(cells)] + [("SF", "k2")]
+    theta = np.array([1.2, 1.21, 1.19, 1.2, 6.0])
+    fk, bk, tk, layout, _ = adjoint.adjoint_kernels(ring, ufo.KICK | ufo.DOUBLE_PRECISION, 24, COORDS, shared,
+                                                    every=8)
+    assert layout["gscale"][-1] == 1.0 and set(layout["gscale"][:cells]) == {0.5 / 8}
+    bunch = lattices.gaussian_bunch(40, seed=5, sx=2e-3, sy=2e-3)
+    _check(ring, ufo.KICK | ufo.DOUBLE_PRECISION, shared, theta, bunch)
