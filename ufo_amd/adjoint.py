@@ -479,12 +479,13 @@ def _reverse_layout(ops, env: Env, ntheta: int):
 
     Gradient scaling: the flush of a sum multiplies it by the derivative of
     its coefficient.  When every derivative is a constant times 1/(1+dp)
-    (kicks in 6D), lambda is carried multiplied by oodppo instead (the
-    recursion lambda <- J^T lambda is linear, so every sum carries the
-    factor) and divided by it again for dL/dx0; when every derivative is
-    then a constant, and one constant serves all sums of a parameter, that
-    constant is applied to the parameter's total after the reduction
-    (``gscale``) and the flush adds the bare sum.
+    (6D kicks on a turn that does not run on scaled momenta), lambda is
+    carried multiplied by oodppo instead (the recursion lambda <- J^T lambda
+    is linear, so every sum carries the factor) and divided by it again for
+    dL/dx0.  When every derivative is then a constant, and one constant
+    serves all sums of a parameter, that constant is applied to the
+    parameter's total after the reduction (``gscale``) and the flush adds
+    the bare sum; otherwise the flush multiplies by the constants.
 
     Scaled momenta: when the turn is drifts, kicks scaled by 1/(1+dp) and
     apertures (``_scalable``), the reverse pass carries p' = p (1 + dp) and
@@ -536,7 +537,7 @@ def _reverse_layout(ops, env: Env, ntheta: int):
         out += segment(ops)
 
     kinds = {t[2][0] if t[2] else None for _, _, _, terms in flushes for t in terms}
-    factored = None not in kinds and kinds <= {"const"} | ({"chi"} if kinds == {"chi"} else set())
+    factored = kinds in (set(), {"const"}, {"chi"})
     gscale = [1.0] * ntheta
     bare = set()      # flushes that add the bare sum
     if factored:
