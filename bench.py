@@ -55,6 +55,8 @@ def main():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-seed-adjoint", action="store_true")
     ap.add_argument("--no-fodo", action="store_true")
+    ap.add_argument("--track-particles", type=int, default=4096,
+                    help="particles of the FODO-16 Track line (4096: the reference's CPU case)")
     ap.add_argument("--no-lossy", action="store_true")
     ap.add_argument("--no-co-optics", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0,
@@ -163,7 +165,8 @@ def main():
         out["lossy_da"] = lossy_scan_timing(args.nx, args.ny, args.turns, args.steps, achieved_tflops)
         _progress("lossy DA scan done")
     if not args.no_fodo:
-        out["fodo_track"] = fodo_track_timing(cpu=not args.no_cpu)
+        out["fodo_track"] = fodo_track_timing(particles=args.track_particles,
+                                              cpu=not args.no_cpu and args.track_particles == 4096)
         _progress("FODO Track done")
     if not args.no_co_optics:
         out["co_optics"] = co_optics_timing(cpu=not args.no_cpu)
@@ -294,7 +297,9 @@ def fodo_track_timing(particles=4096, turns=1000, reps=3, cpu=True):
     out = {"workload": f"16-cell FODO ({E} elements, KICK, 6D), {particles} particles, {turns} turns, where=[-1]",
            "gpu_kernel_ms": round(float(np.median(k_ms)), 3), "gpu_run_ms": round(float(np.median(run_ms)), 3),
            "run_launches": chunks, "gpu_run_one_launch_ms": round(float(np.median(one_ms)), 3),
-           "gpu_pet_per_s": pt * E / (float(np.median(k_ms)) * 1e-3), "survivors": int(alive.sum())}
+           "gpu_pet_per_s": pt * E / (float(np.median(k_ms)) * 1e-3), "survivors": int(alive.sum()),
+           "gpu_tflops": pt * tr.kernel.flops_per_turn / (float(np.median(k_ms)) * 1e-3) / 1e12}
+    out["frac_fp64_peak"] = out["gpu_tflops"] / PEAK_FP64_TFLOPS
     if cpu:
         from oracle import runner
         threads = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
