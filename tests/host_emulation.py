@@ -26,6 +26,7 @@ static ufo_dim3 blockIdx, threadIdx;
 #define __global__
 #define __device__
 #define __launch_bounds__(n)
+#define UFO_STREAM_STORE(v, p) (*(p) = (v))
 """
 
 _DRIVER = r"""

@@ -783,6 +783,9 @@ typedef {ufloat} ufloat;
 #ifndef NAN
 #define NAN UFO_NAN
 #endif
+#ifndef UFO_STREAM_STORE
+#define UFO_STREAM_STORE(v, p) __builtin_nontemporal_store((v), (p))
+#endif
 """
 
 
@@ -804,10 +807,14 @@ class Kernel:
         self.src = crmath.attach(self.src)
 
 
-def _layout(body_ops, env, observe=None, loop_var="rep", emit=None):
+def _layout(body_ops, env, observe=None, loop_var="rep", emit=None, auto_unroll=False):
     """Emit the op list, as a loop over the repeated period when there is
     one.  ``emit(op, env, out, observe)`` writes one op (default: the
-    forward pass)."""
+    forward pass).  With ``auto_unroll`` a short period is unrolled
+    (``unroll_for``): the loop's own instructions and its branch then come
+    once per several repetitions, which matters when a repetition is a few
+    dozen FP64 instructions (a FODO cell) and the wave's operations form one
+    dependent chain."""
     emit = emit or emit_op
     remap_from = env.shared_offset if (env.remap_loops and env.table is not None) else None
     head, p, reps, stride = (find_period(body_ops, True, remap_from) if len(body_ops) > 8 and not env.no_loop
@@ -825,6 +832,8 @@ def _layout(body_ops, env, observe=None, loop_var="rep", emit=None):
         except NeedsFlat:
             head, p, reps, stride = 0, 0, 0, 0
     env.last_period = (head, p, reps, stride)
+    if auto_unroll and reps >= 2:
+        env.unroll = unroll_for(body_ops[head:head + p])
     out: List[str] = []
     if reps >= 2:
         for op in body_ops[:head]:
@@ -983,7 +992,7 @@ def sa_kernel(line, flags, turns, parameters, dp, block=256, name="ufo_sa") -> K
     ops = lower(line, flags, parameters)
     src = _coord_sources(parameters, flags, dp)
     env = _env_for(flags, src, ops)
-    body, period = _layout(ops, env)
+    body, period = _layout(ops, env, auto_unroll=True)
     P = max(1, len(parameters))
     one = env.lit(1.0)
     load = _load_block(env, src, P, "row", "            ")
@@ -1055,10 +1064,13 @@ def track_kernel(line, flags, turns, parameters, where, dp, block=256, name="ufo
     dumps = sum(1 for op in ops if isinstance(op, ir.Observe))
     src = _coord_sources(parameters, flags, dp)
     env = _env_for(flags, src, ops)
+    # samples are written once and read back only by the copy to the host:
+    # streaming (non-temporal) stores
     store = ("{ size_t b = (size_t)s * 6 * np + idx; "
-             "tracks[b] = x; tracks[b + np] = px; tracks[b + 2 * np] = y; "
-             "tracks[b + 3 * np] = py; tracks[b + 4 * np] = z; tracks[b + 5 * np] = dp; ++s; }")
-    body, period = _layout(ops, env, observe=store)
+             "UFO_STREAM_STORE(x, tracks + b); UFO_STREAM_STORE(px, tracks + b + np); "
+             "UFO_STREAM_STORE(y, tracks + b + 2 * np); UFO_STREAM_STORE(py, tracks + b + 3 * np); "
+             "UFO_STREAM_STORE(z, tracks + b + 4 * np); UFO_STREAM_STORE(dp, tracks + b + 5 * np); ++s; }")
+    body, period = _layout(ops, env, observe=store, auto_unroll=True)
     P = max(1, len(parameters))
     load = _load_block(env, src, P, "row", "    ")
     code = _PRELUDE.format(ufloat="float" if env.fp32 else "double",
