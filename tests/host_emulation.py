@@ -63,15 +63,16 @@ def compile_track(kernel):
     return ctypes.CDLL(so)
 
 
-def run_track(kernel, pool, samples, splits=()):
+def run_track(kernel, pool, samples, splits=(), fill=np.nan):
     """Host run of a generated Track kernel: returns [particles, samples, 6].
     ``splits`` are the turns at which the run is cut into launches that
-    resume from the saved state, as Track.run does."""
+    resume from the saved state, as Track.run does; ``fill`` is the value
+    the samples buffer holds before the first launch."""
     lib = compile_track(kernel)
     dt = np.float32 if kernel.fp32 else np.float64
     pool = np.ascontiguousarray(pool, dtype=dt)
     n = pool.shape[0]
-    out = np.full((samples, 6, n), np.nan, dtype=dt)
+    out = np.full((samples, 6, n), fill, dtype=dt)
     state = np.full((4, max(1, n)), np.nan, dtype=dt)
     bounds = [0] + sorted(splits) + [kernel.turns]
     for t0, t1 in zip(bounds[:-1], bounds[1:]):

@@ -60,9 +60,10 @@ def test_track_split_over_launches_is_bit_identical(splits):
     """Track.run splits the turns over launches that resume from the saved
     (x, px, y, py); on the host build of the same kernel any split gives the
     one-launch result bit for bit: particles lost at the aperture part-way
-    (their later samples stay unwritten), 6D with dp per particle, and a
-    fractional observation point on a thin element that is never written
-    (sample numbering turn * dumps_per_turn + k, the tail left unwritten)."""
+    (the kernel writes NaN into their later samples), 6D with dp per
+    particle, and a fractional observation point on a thin element that is
+    never written (sample numbering turn * dumps_per_turn + k, the tail left
+    unwritten by the kernel)."""
     ring = lattices.fodo_ring(16)
     bunch = lattices.gaussian_bunch(100, seed=9)
     bunch[:6, 0] = [0.035, 0.038, 0.045, 0.05, 0.0399, -0.039]
@@ -76,6 +77,13 @@ def test_track_split_over_launches_is_bit_identical(splits):
     lost = np.isnan(one[:, 119, 0])              # the last written sample
     assert 0 < lost.sum() < len(bunch) and np.all(np.isnan(one[:, 120:, :]))
     np.testing.assert_array_equal(split.view(np.uint64), one.view(np.uint64))
+    # the kernel itself writes NaN into every sample of its turns after a
+    # loss (the driver fills only the never-written tail): from a buffer of
+    # 7.0, every one of the 120 numbered samples is written, the tail is not
+    for cut in ((), splits):
+        mark = host_emulation.run_track(k, bunch, samples, splits=cut, fill=7.0)
+        np.testing.assert_array_equal(mark[:, :120].view(np.uint64), one[:, :120].view(np.uint64))
+        assert np.all(mark[:, 120:] == 7.0)
 
 
 def test_observation_point_in_every_cell_of_a_periodic_ring():

@@ -1053,9 +1053,10 @@ def track_kernel(line, flags, turns, parameters, where, dp, block=256, name="ufo
 
     One launch runs turns [turn0, turn1).  A launch that starts after turn 0
     resumes from ``state`` ([4][particles]: x, px, y, py; z and dp never
-    change and are loaded from the pool as at turn 0) and returns at once for
+    change and are loaded from the pool as at turn 0) and tracks nothing for
     a particle already lost; a launch that ends before the last turn saves the
-    state.  Splitting the turns over launches changes no result, and lets the
+    state.  The launch writes every sample of its turns: those after the
+    particle's loss turn are NaN (the reference leaves them unwritten).  Splitting the turns over launches changes no result, and lets the
     caller copy finished samples back while later turns run."""
     where = list(where) if isinstance(where, (list, tuple)) else [where]
     ops = lower(line, flags, parameters, where)
@@ -1086,16 +1087,29 @@ extern "C" __global__ void __launch_bounds__({block})
     const size_t np = (size_t)particles;
     const ufloat *__restrict__ row = pool + (size_t)idx * {P};
 {chr(10).join(load)}
+    bool alive = true;
     if (turn0 > 0) {{
         x = state[idx]; px = state[np + idx]; y = state[2 * np + idx]; py = state[3 * np + idx];
-        if (isnan(x)) return;
+        alive = !isnan(x);
     }}
     int s = turn0 * {dumps};
-    for (int turn = turn0; turn < turn1; ++turn) {{
-{chr(10).join('        ' + s_ for s_ in body)}
-        if (isnan(x)) break;
+    if (alive) {{
+        for (int turn = turn0; turn < turn1; ++turn) {{
+{chr(10).join('            ' + s_ for s_ in body)}
+            if (isnan(x)) break;
+        }}
     }}
-    if (turn1 < {turns}) {{
+    // the samples of this launch after the particle's loss turn are NaN
+    if (s < turn1 * {dumps}) {{
+        const ufloat q = UFO_NAN;
+        for (; s < turn1 * {dumps}; ++s) {{
+            size_t b = (size_t)s * 6 * np + idx;
+            UFO_STREAM_STORE(q, tracks + b); UFO_STREAM_STORE(q, tracks + b + np);
+            UFO_STREAM_STORE(q, tracks + b + 2 * np); UFO_STREAM_STORE(q, tracks + b + 3 * np);
+            UFO_STREAM_STORE(q, tracks + b + 4 * np); UFO_STREAM_STORE(q, tracks + b + 5 * np);
+        }}
+    }}
+    if (alive && turn1 < {turns}) {{
         state[idx] = x; state[np + idx] = px; state[2 * np + idx] = y; state[3 * np + idx] = py;
     }}
 }}

@@ -75,11 +75,20 @@ class Track(Simulation):
         self.module().launch(self.kernel.name, grid, block,
                              [self._pool_dev, self._tracks_dev, self._state_dev, self.particles, turn0, turn1])
 
+    def _fill_unwritten(self):
+        """NaN into the samples no launch writes: observation points past
+        turns * dumps_per_turn (fractional points on thin elements, never
+        observed, as in the reference).  The kernel writes every other
+        sample, NaN after a particle's loss."""
+        first = self.turns * self.kernel.dumps_per_turn
+        if first < self._tracks_dev.shape[0]:
+            self._tracks_dev[first:].fill_(float("nan"))
+
     def launch(self):
         """Enqueue one tracking pass on the current stream with the device
         pool as input (no host copies)."""
         self._buffers()
-        self._tracks_dev.fill_(float("nan"))
+        self._fill_unwritten()
         if self.particles == 0:
             return
         self._launch_turns(0, self.turns)
@@ -102,7 +111,7 @@ class Track(Simulation):
         n = self._chunks()
         if self.particles == 0 or n == 1:
             self._pool_dev.copy_(self._host_pool(), non_blocking=True)
-            self._tracks_dev.fill_(float("nan"))
+            self._fill_unwritten()
             if self.particles:
                 self._launch_turns(0, self.turns)
             self._tracks_host.copy_(self._tracks_dev, non_blocking=True)
@@ -121,7 +130,7 @@ class Track(Simulation):
         d = self.kernel.dumps_per_turn
         with torch.cuda.stream(runner):
             self._pool_dev.copy_(self._host_pool(), non_blocking=True)
-            self._tracks_dev.fill_(float("nan"))
+            self._fill_unwritten()
             for k, (t0, t1) in enumerate(zip(bounds[:-1], bounds[1:])):
                 self._launch_turns(t0, t1)
                 done = self._events[k]
