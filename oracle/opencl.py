@@ -468,10 +468,3 @@ def regular_mask(variants, turns, tol=1e-9, leave_out=()):
             d = np.abs(np.nan_to_num(state - ref_state, nan=0.0)) / scale
         regular &= ~alive | (np.max(d, axis=1) <= tol)
     return regular
-
-
-def loss_turn_regular(line, pool, flags, turns, parameters, threads, tol=1e-9, dp=0.0):
-    """(loss turns of the default build, regular mask over all six builds of
-    ``reference_variants``, default build ms)."""
-    v = reference_variants(line, pool, flags, turns, parameters, threads, dp)
-    return v["default"][0], regular_mask(v, turns, tol), v["default"][2]

@@ -18,7 +18,6 @@ differs, and each such divergence has a test of exactly that difference:
 * A Line in an unknown style: the reference fails on an unbound local;
   here ValueError.
 """
-import os
 
 import pytest
 

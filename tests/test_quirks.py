@@ -5,7 +5,7 @@ import numpy as np
 import pytest
 
 import ufo_amd as ufo
-from ufo_amd import codegen, ir, lattices
+from ufo_amd import codegen, lattices
 from oracle import runner
 
 import host_emulation
