@@ -77,6 +77,29 @@ def test_fodo_track_6d():
     assert np.all(gpu[alive, :, 4] == 0.0)
 
 
+def test_track_launch_overwrites_a_stale_samples_buffer():
+    """Track.launch() on a samples buffer holding stale values (7.0): the
+    kernel writes every sample of its turns, NaN after a particle's loss,
+    and the driver fills the never-written tail, so the result equals run()
+    bit for bit (particles lost part-way, a fractional point on a thin
+    element)."""
+    ring = lattices.fodo_ring(16)
+    bunch = lattices.gaussian_bunch(300, seed=3)
+    bunch[:6, 0] = [0.035, 0.038, 0.045, 0.05, 0.0399, -0.039]
+    sf = ring.find(lambda e: isinstance(e, ufo.Multipole))[0]
+    tr = ufo.Track(ring, flags=ufo.KICK | ufo.DOUBLE_PRECISION, turns=200, particles=len(bunch),
+                   parameters=COORD6, where=[2.5, sf + 0.5, -1])
+    tr.parameters[:] = bunch
+    tr.run()
+    want = np.array(tr.tracks)
+    lost = np.isnan(want[:, 399, 0])
+    assert 0 < lost.sum() < len(bunch) and np.all(np.isnan(want[:, 400:, :]))
+    tr._tracks_dev.fill_(7.0)
+    tr.launch()
+    got = tr.tracks_device.cpu().numpy()
+    np.testing.assert_array_equal(got.view(np.uint64), want.view(np.uint64))
+
+
 @pytest.mark.parametrize("double", [True, False])
 def test_track_run_split_over_launches(double):
     """Track.run splits the turns over launches (copying finished samples
