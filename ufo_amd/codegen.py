@@ -1056,8 +1056,9 @@ def track_kernel(line, flags, turns, parameters, where, dp, block=256, name="ufo
     change and are loaded from the pool as at turn 0) and tracks nothing for
     a particle already lost; a launch that ends before the last turn saves the
     state.  The launch writes every sample of its turns: those after the
-    particle's loss turn are NaN (the reference leaves them unwritten).  Splitting the turns over launches changes no result, and lets the
-    caller copy finished samples back while later turns run."""
+    particle's loss turn are NaN (the reference leaves them unwritten).
+    Splitting the turns over launches changes no result, and lets the caller
+    copy finished samples back while later turns run."""
     where = list(where) if isinstance(where, (list, tuple)) else [where]
     ops = lower(line, flags, parameters, where)
     if -1 in where:
