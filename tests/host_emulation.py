@@ -92,7 +92,7 @@ int main() {
     ufloat *pool = (ufloat *)malloc(sizeof(ufloat) * n * P);
     ufloat *tracks = (ufloat *)malloc(sizeof(ufloat) * S * 6 * n);
     ufloat *state = (ufloat *)malloc(sizeof(ufloat) * 4 * n);
-    for (int i = 0; i < n * P; ++i) pool[i] = (ufloat)(1e-4 * ((i * 7919) %% 101 - 50));
+    for (int i = 0; i < n * P; ++i) pool[i] = (ufloat)(%(scale)r * ((i * 7919) %% 101 - 50));
     for (int i = 0; i < S * 6 * n; ++i) tracks[i] = 0;
     const int b[] = {%(bounds)s};
     for (int k = 0; k + 1 < (int)(sizeof b / sizeof b[0]); ++k)
@@ -103,15 +103,16 @@ int main() {
 """
 
 
-def sanitized_run(kernel, particles, columns, splits=()):
+def sanitized_run(kernel, particles, columns, splits=(), scale=1e-4):
     """Build the emitted kernel, the launch loop and exactly sized buffers
     as a host program under AddressSanitizer and UndefinedBehaviorSanitizer
-    and run it over the turns cut at ``splits``; returns (exit status,
-    sanitizer report)."""
+    and run it over the turns cut at ``splits``; pool entries are ``scale``
+    times integers in [-50, 50]; returns (exit status, sanitizer report)."""
     samples = kernel.dumps_per_turn * kernel.turns
     bounds = ", ".join(str(t) for t in [0] + sorted(splits) + [kernel.turns])
     src = (_PRELUDE + kernel.src + (_DRIVER % kernel.name)
-           + _SANITIZED_MAIN % dict(n=particles, P=max(1, columns), S=samples, block=kernel.block, bounds=bounds))
+           + _SANITIZED_MAIN % dict(n=particles, P=max(1, columns), S=samples, block=kernel.block, bounds=bounds,
+                                    scale=float(scale)))
     d = tempfile.mkdtemp(prefix="ufo_sanitized_")
     cpp, exe = os.path.join(d, "k.cpp"), os.path.join(d, "k")
     with open(cpp, "w") as f:

@@ -102,23 +102,28 @@ def test_observation_point_in_every_cell_of_a_periodic_ring():
     assert_parity(r)
 
 
-@pytest.mark.parametrize("flags,params,where,turns,splits,n", [
-    (ufo.KICK | ufo.DOUBLE_PRECISION, ["x", "px", "y", "py", "dp"], [-1], 20, [], 77),
-    (ufo.KICK | ufo.DOUBLE_PRECISION, ["x", "px", "y", "py", "dp"], [0, 3, 9, -1], 20, [7, 13], 300),
-    (ufo.KICK | ufo.DOUBLE_PRECISION | ufo.FIVED, ["x", "y"], [2, -1], 9, [1, 8], 65),
-    (ufo.KICK | ufo.EXACT | ufo.DOUBLE_PRECISION, ["x", "px", "y", "py", "dp"], [-1], 10, [5], 1),
-    (ufo.KICK, ["x", "px", "y", "py", "dp"], [1, -1], 10, [3], 129),
-    (ufo.DOUBLE_PRECISION, ["x", "px", "y", "py", "dp"], [-1, 4], 12, [6], 64),
-    (ufo.KICK | ufo.DOUBLE_PRECISION, [], [-1], 5, [], 4),
+@pytest.mark.parametrize("flags,params,where,turns,splits,n,scale", [
+    (ufo.KICK | ufo.DOUBLE_PRECISION, ["x", "px", "y", "py", "dp"], [-1], 20, [], 77, 1e-4),
+    (ufo.KICK | ufo.DOUBLE_PRECISION, ["x", "px", "y", "py", "dp"], [0, 3, 9, -1], 20, [7, 13], 300, 1e-4),
+    (ufo.KICK | ufo.DOUBLE_PRECISION | ufo.FIVED, ["x", "y"], [2, -1], 9, [1, 8], 65, 1e-4),
+    (ufo.KICK | ufo.EXACT | ufo.DOUBLE_PRECISION, ["x", "px", "y", "py", "dp"], [-1], 10, [5], 1, 1e-4),
+    (ufo.KICK, ["x", "px", "y", "py", "dp"], [1, -1], 10, [3], 129, 1e-4),
+    (ufo.DOUBLE_PRECISION, ["x", "px", "y", "py", "dp"], [-1, 4], 12, [6], 64, 1e-4),
+    (ufo.KICK | ufo.DOUBLE_PRECISION, [], [-1], 5, [], 4, 1e-4),
+    (ufo.KICK | ufo.DOUBLE_PRECISION, ["x", "px", "y", "py", "dp"], [0, 3, -1], 20, [7, 13], 300, 2e-4),
+    (ufo.KICK | ufo.DOUBLE_PRECISION | ufo.FIVED, ["x", "y"], [2, -1], 9, [1, 8], 65, 8e-4),
 ])
-def test_emitted_track_kernel_stays_in_bounds(flags, params, where, turns, splits, n):
+def test_emitted_track_kernel_stays_in_bounds(flags, params, where, turns, splits, n, scale):
     """The emitted Track kernel and its resumable launches, built for the
     host under AddressSanitizer and UndefinedBehaviorSanitizer with every
     buffer exactly as large as the launch needs: particle counts that do
     not fill a block, several observation points, launches cut at arbitrary
-    turns, FIVED, EXACT, thick, FP32 and no parameter columns."""
+    turns, FIVED, EXACT, thick, FP32 and no parameter columns; with larger
+    pool values (scale 2e-4 in 6D: 168 of 300 particles lost over the turns;
+    8e-4 in FIVED: 39 of 65, 13 in the first turn) the NaN samples after a
+    loss are written too, also by launches that resume a lost particle."""
     k = codegen.track_kernel(lattices.fodo_ring(4), flags, turns, params, where, 0.0)
-    rc, report = host_emulation.sanitized_run(k, n, len(params), splits)
+    rc, report = host_emulation.sanitized_run(k, n, len(params), splits, scale=scale)
     assert rc == 0, report[-2000:]
 
 
